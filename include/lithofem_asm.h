@@ -1,0 +1,170 @@
+/*
+ * lithofem_asm.h -- C-ABI of the MI355X (gfx950) Nedelec system-matrix
+ * assembly.  Plain C: opaque handle, plain pointers and sizes, int status.
+ *
+ * Every entry point below replaces a member of the reference's CUDA
+ * assembly classes AsmGpu / AsmGpuGlobal (solver/asm_gpu.inc:271-599), which
+ * the reference's solve driver calls at solver/lithofem_solve.cpp:783-792
+ * (Init, BuildGlobal, ComputeLocal, ScatterEliminate), :873-875 (device CSR
+ * handoff) and :910 (Residual).  The driver-side shim fills lfasm_mesh from
+ * the reference's AsmData (solver/asm_data.inc:77-97) and Problem
+ * (solver/problem_common.inc:40-65) exactly as AsmGpu::Init reads them.
+ *
+ * Status codes: 0 = OK; negative = error class (LFASM_E_*).  After an error
+ * the handle's lfasm_last_error() string says what failed; the caller is
+ * expected to fall back to its CPU assembly, as the reference driver does
+ * (solver/lithofem_solve.cpp:796-799).
+ *
+ * Threading: one handle per host thread; each handle owns one HIP stream on
+ * its device and every device call of the handle is ordered on it.  Calls
+ * that return host data synchronise that stream.
+ */
+#ifndef LITHOFEM_ASM_H
+#define LITHOFEM_ASM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+   LFASM_OK = 0,
+   LFASM_E_ARG = -1,      /* invalid argument or size */
+   LFASM_E_HIP = -2,      /* HIP runtime error (message in last_error) */
+   LFASM_E_FORCED = -3,   /* LITHOFEM_ASM_FORCE_FAIL set in the environment */
+   LFASM_E_UNSUPPORTED = -4, /* element order or size outside the kernels */
+   LFASM_E_STATE = -5,    /* call order (e.g. scatter before build_global) */
+   LFASM_E_PATTERN = -6   /* an element (row, col) pair is missing from the CSR */
+};
+
+/* integrator selection bits, as the reference's mask enum
+ * (solver/asm_gpu.inc:51-58) */
+enum {
+   LFASM_CURLCURL = 1,
+   LFASM_MASS = 2,
+   LFASM_CROSS0 = 4,
+   LFASM_CROSS1 = 8,
+   LFASM_KLK = 16
+};
+
+typedef struct lfasm_handle lfasm_handle;
+
+/* One reference integration rule (solver/asm_data.inc:38-45). */
+typedef struct {
+   int nq;
+   const double *ip;       /* nq*3 reference points */
+   const double *w;        /* nq weights */
+   const double *vshape;   /* nq*nd*3, [(q*nd + k)*3 + d] */
+   const double *curl;     /* nq*nd*3 */
+} lfasm_rule;
+
+/* PML and Bloch parameters: the fields of the reference's AsmPmlParams
+ * (solver/asm_gpu.inc:31-36), smax already per side. */
+typedef struct {
+   double z_min, z_max, thick, order;
+   double smax_bot, smax_top;
+   double k0, kx, ky;
+} lfasm_pml;
+
+/* Mesh + discretisation data (solver/asm_data.inc:77-97 plus the region eps
+ * table and PML parameters AsmGpu::Init takes from prob,
+ * solver/asm_gpu.inc:296-319).  All arrays are host pointers, copied during
+ * lfasm_init. */
+typedef struct {
+   int ne, nd, p, height;
+   const int *dof;          /* ne*nd decoded global dof (d<0 -> -1-d) */
+   const double *sign;      /* ne*nd  +/-1 */
+   const int *fo;           /* ne*4 face orientations 0..5 (p >= 2; else NULL) */
+   const int *attr;         /* ne 1-based region attribute */
+   const double *J;         /* ne*9 row-major Jacobian */
+   const double *Jinv;      /* ne*9 row-major inverse Jacobian */
+   const double *detJ;      /* ne */
+   const double *v0;        /* ne*3 element origin */
+   lfasm_rule rule_cc;      /* CurlCurl rule (order 2p-2) */
+   lfasm_rule rule_mass;    /* mass / mixed rule (order 2p) */
+   int nreg;
+   const double *eps_tab;   /* nreg interleaved complex, by attribute-1 */
+   const double *eps_elem;  /* optional ne interleaved complex (overrides eps_tab) */
+   lfasm_pml pml;
+   const double *TInv;      /* 24 doubles: solver/asm_data.inc:26-34 layout */
+} lfasm_mesh;
+
+/* ---- lifetime (AsmGpu constructor / Release, solver/asm_gpu.inc:368-376) */
+int lfasm_create(int device, lfasm_handle **out);
+void lfasm_destroy(lfasm_handle *h);
+const char *lfasm_last_error(const lfasm_handle *h);
+
+/* smax = -ln(target) (order+1) / (2 k0 n_med thick)
+ * (solver/asm_gpu.inc:316-319); helper for the driver-side shim. */
+double lfasm_pml_smax(double target, double order, double k0, double n_med, double thick);
+
+/* AsmGpu::Init (solver/asm_gpu.inc:276-322): upload geometry, tables,
+ * coefficients; allocate the local-matrix buffer.  Honours
+ * LITHOFEM_ASM_FORCE_FAIL (returns LFASM_E_FORCED before any device call). */
+int lfasm_init(lfasm_handle *h, const lfasm_mesh *m);
+
+/* Per-element complex eps (no reference counterpart: the reference reads
+ * region eps only, solver/asm_gpu.inc:96).  eps = ne interleaved complex, or
+ * NULL to return to the region table. */
+int lfasm_set_eps_elem(lfasm_handle *h, const double *eps);
+
+/* AsmGpuGlobal::BuildGlobal (solver/asm_gpu.inc:501-514): CSR pattern
+ * (I: height+1, Jcol: nnz, sorted columns) and essential mask (height
+ * bytes, 1 = essential).  Also builds the element-to-slot map and the
+ * per-row contributor lists of the deterministic scatter. */
+int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t nnz,
+                       const unsigned char *is_ess);
+
+/* AsmGpu::ComputeLocal (solver/asm_gpu.inc:326-351): element matrices of
+ * the CurlCurlIntegrator (mask_cc & LFASM_CURLCURL) on the 2p-2 rule plus
+ * the masked mass / mixed / KLK integrators (mask_mass) on the 2p rule,
+ * then (apply_te, p >= 2) the dual face-DOF transform.  Ae is overwritten. */
+int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te);
+
+/* AsmGpuGlobal::ScatterEliminate (solver/asm_gpu.inc:517-530): CSR values
+ * from the local matrices by a deterministic per-row segmented reduction
+ * (contributions summed in ascending element order), then (eliminate)
+ * essential rows/columns -> 0 with unit diagonal. */
+int lfasm_scatter_eliminate(lfasm_handle *h, int eliminate);
+
+/* AsmGpuGlobal::DeviceI / DeviceJ / DeviceValues (solver/asm_gpu.inc:
+ * 579-583): borrowed device pointers (values interleaved complex double). */
+int lfasm_device_csr(lfasm_handle *h, int **I, int **Jcol, void **vals);
+
+/* Production download of the CSR values for the host direct solver, split
+ * re/im in (I, Jcol) order (the layout of MFEM's ComplexSparseMatrix
+ * real()/imag() after SortColumnIndices). */
+int lfasm_download_values(lfasm_handle *h, double *re, double *im);
+
+/* AsmGpu::DownloadLocal (solver/asm_gpu.inc:355-362): test-only D2H of the
+ * ne*nd*nd interleaved local matrices; prints the marker line
+ * "asm_gpu_test_download (test-only D2H)". */
+int lfasm_download_local(lfasm_handle *h, double *Ae);
+
+/* AsmGpuGlobal::Residual (solver/asm_gpu.inc:533-565): ||A X - B|| / ||B||
+ * with X, B as 2n [re; im] host vectors. */
+int lfasm_residual(lfasm_handle *h, const double *X2n, const double *B2n, double *rel);
+
+/* n(), nnz(), ne(), nd() of the reference classes. */
+int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
+
+/* Device-event timings (milliseconds) of the last calls, in this order:
+ * [0] compute_local kernels, [1] scatter_eliminate kernels,
+ * [2] build_global device part, [3] last batched call, [4] last adjoint call. */
+int lfasm_timings(const lfasm_handle *h, double *ms, int n);
+
+/* The handle's HIP stream (hipStream_t) for callers that record their own
+ * events or order their own work after the assembly. */
+void *lfasm_stream(lfasm_handle *h);
+
+/* build_csr_symbolic (solver/asm_data.inc:178-210), host C++ (multi-thread):
+ * I (height+1) is filled; *Jcol is allocated with malloc and must be freed
+ * with lfasm_free.  Returns nnz or a negative status. */
+int64_t lfasm_csr_symbolic(int ne, int nd, int height, const int *dof, int *I, int **Jcol);
+void lfasm_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,682 @@
+// Device code of the gfx950 Nedelec assembly: element matrices (MFMA FP64
+// for 16-wide tiles, VALU for the generic path), dual face-DOF transform,
+// deterministic row-segmented scatter with fused essential elimination,
+// CSR slot map, SpMV.
+//
+// Mathematical content follows the reference assembly
+// (solver/asm_gpu.inc:70-214 local matrices, :221-251 face transform,
+// :434-478 scatter/elimination, :481-493 SpMV; coefficient definitions in
+// solver/problem_common.inc:69-98,150-216).  The element matrix is computed
+// as the quadrature contraction Ae = B^T C, where B stacks the physical
+// shape components of every basis function at every quadrature point
+// (K rows) and C = (w D) B applies the per-point complex coefficient
+// matrix D of the five integrators; the K stream is processed in chunks of
+// 12 rows staged in LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lfasm {
+
+constexpr int BLOCK = 256;        // 4 wavefronts of 64 lanes
+constexpr int CHUNK_ROWS = 12;    // K rows staged per chunk (3 MFMA k-steps)
+
+struct RuleDev {
+   int nq;
+   const double *ip, *w, *vs, *cs;
+};
+
+struct PmlDev {
+   double z_min, z_max, thick, order, smax_bot, smax_top, k0, kx, ky;
+};
+
+struct LocalParams {
+   int ne, nd, p;
+   const double *J, *Ji, *detJ, *v0;
+   const int *attr;
+   const double2 *eps_tab;
+   const double2 *eps_elem;
+   const int *fo;
+   RuleDev rc, rm;
+   PmlDev pml;
+   double TInv[24];
+   int use_cc;       // curl-curl stream on the cc rule
+   int mflags;       // bit0 mass, bit1 cross0, bit2 cross1, bit3 klk
+   int apply_te;
+   double2 *Ae;
+};
+
+// ---- complex helpers ------------------------------------------------------
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// 1/s as conj(s)/|s|^2 (the reference device path, solver/asm_gpu.inc:23-27)
+__device__ __forceinline__ double2 cinv(double2 a)
+{
+   const double d = a.x * a.x + a.y * a.y;
+   return make_double2(a.x / d, -a.y / d);
+}
+
+// PML stretch s(z) (solver/asm_gpu.inc:40-48)
+__device__ __forceinline__ double2 pml_s(const PmlDev &p, double z)
+{
+   double d = -1.0, smax = 0.0;
+   if (z < p.z_min) { d = (p.z_min - z) / p.thick; smax = p.smax_bot; }
+   else if (z > p.z_max) { d = (z - p.z_max) / p.thick; smax = p.smax_top; }
+   if (d < 0.0) { return make_double2(1.0, 0.0); }
+   if (d > 1.0) { d = 1.0; }
+   return make_double2(1.0, smax * pow(d, p.order));
+}
+
+// ---- per-element coefficient setup ----------------------------------------
+// Element-uniform data of one element, loaded by every lane of the group.
+struct ElemGeo {
+   double Jd[9];    // J / detJ (contravariant Piola)
+   double Ji[9];    // row-major J^{-1}
+   double J6, J7, J8, v0z, detJ;
+   double2 eps;
+};
+
+__device__ __forceinline__ void load_geo(const LocalParams &P, int e, ElemGeo &g)
+{
+   const double det = P.detJ[e];
+   const double inv = 1.0 / det;
+#pragma unroll
+   for (int i = 0; i < 9; ++i)
+   {
+      g.Jd[i] = P.J[(size_t)e * 9 + i] * inv;
+      g.Ji[i] = P.Ji[(size_t)e * 9 + i];
+   }
+   g.J6 = P.J[(size_t)e * 9 + 6];
+   g.J7 = P.J[(size_t)e * 9 + 7];
+   g.J8 = P.J[(size_t)e * 9 + 8];
+   g.v0z = P.v0[(size_t)e * 3 + 2];
+   g.detJ = det;
+   g.eps = P.eps_elem ? P.eps_elem[e] : P.eps_tab[P.attr[e] - 1];
+}
+
+// Per quadrature point q (cc rule first, then mass rule): w*Lam^-1 diagonal
+// factors wa = w/s and wb = w*s, with w = weight*detJ.
+__device__ __forceinline__ void q_coef(const LocalParams &P, const ElemGeo &g, int q,
+                                       double *qc)
+{
+   const RuleDev &R = (q < P.rc.nq) ? P.rc : P.rm;
+   const int qq = (q < P.rc.nq) ? q : q - P.rc.nq;
+   const double *xi = R.ip + (size_t)qq * 3;
+   const double z = g.v0z + g.J6 * xi[0] + g.J7 * xi[1] + g.J8 * xi[2];
+   const double2 s = pml_s(P.pml, z);
+   const double2 si = cinv(s);
+   const double w = R.w[qq] * g.detJ;
+   qc[0] = w * si.x;
+   qc[1] = w * si.y;
+   qc[2] = w * s.x;
+   qc[3] = w * s.y;
+}
+
+// Row counts of the two K streams.
+struct StreamShape {
+   int nq_cc, nq_m;      // quadrature points used per stream (0 if off)
+   int rows_m;           // rows per mass-rule point: 3 (v only) or 6 (v and curl)
+   int qpc_cc, qpc_m;    // points per 12-row chunk
+   int nch_cc, nch_m;    // chunks per stream
+};
+
+__host__ __device__ inline StreamShape stream_shape(int use_cc, int mflags, int nq_cc,
+                                                    int nq_m)
+{
+   StreamShape s;
+   s.nq_cc = use_cc ? nq_cc : 0;
+   s.nq_m = (mflags & 15) ? nq_m : 0;
+   s.rows_m = (mflags & 6) ? 6 : 3;
+   s.qpc_cc = 4;
+   s.qpc_m = CHUNK_ROWS / s.rows_m;
+   s.nch_cc = (s.nq_cc + s.qpc_cc - 1) / s.qpc_cc;
+   s.nch_m = (s.nq_m + s.qpc_m - 1) / s.qpc_m;
+   return s;
+}
+
+// Mass-rule row coefficients (element constants).  With flags fm (mass),
+// f0 (cross0), f1 (cross1), fk (klk) the C rows of one point are
+//   C0 = wb ((fm me + fk ky^2) v0 - fk kx ky v1 + i f0 ky c2)
+//   C1 = wb ((fm me + fk kx^2) v1 - fk kx ky v0 - i f0 kx c2)
+//   C2 = wa ((fm me + fk (kx^2+ky^2)) v2 - i f0 ky c0 + i f0 kx c1)
+//   C3 = wa (i f1 ky v2)
+//   C4 = wa (-i f1 kx v2)
+//   C5 = wb (-i f1 ky v0 + i f1 kx v1)
+// with me = -k0^2 eps, wa = w/s, wb = w s: the mass, MixedVectorCurl,
+// MixedVectorWeakCurl and -K Lam^-1 K terms of solver/asm_gpu.inc:140-203
+// (K = [[0,0,ky],[0,0,-kx],[-ky,kx,0]], Lam^-1 = diag(1/s, 1/s, s)).
+struct MassCoef {
+   double m0r, m0i, m1r, m1i, m2r, m2i, kxy, c0x, c0y, c1x, c1y;
+};
+
+__device__ __forceinline__ MassCoef mass_coef(const LocalParams &P, double2 eps)
+{
+   const double fm = (P.mflags & 1) ? 1.0 : 0.0;
+   const double f0 = (P.mflags & 2) ? 1.0 : 0.0;
+   const double f1 = (P.mflags & 4) ? 1.0 : 0.0;
+   const double fk = (P.mflags & 8) ? 1.0 : 0.0;
+   const double kx = P.pml.kx, ky = P.pml.ky;
+   const double k2 = P.pml.k0 * P.pml.k0;
+   const double mer = -k2 * eps.x * fm, mei = -k2 * eps.y * fm;
+   MassCoef c;
+   c.m0r = mer + fk * ky * ky;
+   c.m0i = mei;
+   c.m1r = mer + fk * kx * kx;
+   c.m1i = mei;
+   c.m2r = mer + fk * (kx * kx + ky * ky);
+   c.m2i = mei;
+   c.kxy = fk * kx * ky;
+   c.c0x = f0 * kx;
+   c.c0y = f0 * ky;
+   c.c1x = f1 * kx;
+   c.c1y = f1 * ky;
+   return c;
+}
+
+// Form one (point, basis k) item of a chunk into the LDS staging rows
+// A[row][k], Cr[row][k], Ci[row][k] (row stride NP).  qc = per-point
+// factors (wa.re, wa.im, wb.re, wb.im).  valid == false writes zero rows.
+__device__ __forceinline__ void form_cc_item(const LocalParams &P, const ElemGeo &g,
+                                             const double *qc, int q, int k, bool valid,
+                                             double *A, double *Cr, double *Ci, int row0,
+                                             int NP)
+{
+   double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+   if (valid && k < P.nd)
+   {
+      const double *cs = P.rc.cs + ((size_t)q * P.nd + k) * 3;
+      const double h0 = cs[0], h1 = cs[1], h2 = cs[2];
+      c0 = g.Jd[0] * h0 + g.Jd[1] * h1 + g.Jd[2] * h2;
+      c1 = g.Jd[3] * h0 + g.Jd[4] * h1 + g.Jd[5] * h2;
+      c2 = g.Jd[6] * h0 + g.Jd[7] * h1 + g.Jd[8] * h2;
+   }
+   const double war = valid ? qc[0] : 0.0, wai = valid ? qc[1] : 0.0;
+   const double wbr = valid ? qc[2] : 0.0, wbi = valid ? qc[3] : 0.0;
+   A[(row0 + 0) * NP + k] = c0;
+   A[(row0 + 1) * NP + k] = c1;
+   A[(row0 + 2) * NP + k] = c2;
+   Cr[(row0 + 0) * NP + k] = war * c0;
+   Ci[(row0 + 0) * NP + k] = wai * c0;
+   Cr[(row0 + 1) * NP + k] = war * c1;
+   Ci[(row0 + 1) * NP + k] = wai * c1;
+   Cr[(row0 + 2) * NP + k] = wbr * c2;
+   Ci[(row0 + 2) * NP + k] = wbi * c2;
+}
+
+__device__ __forceinline__ void form_mass_item(const LocalParams &P, const ElemGeo &g,
+                                               const MassCoef &mc, const double *qc,
+                                               int q, int k, bool valid, int rows,
+                                               double *A, double *Cr, double *Ci,
+                                               int row0, int NP)
+{
+   double v0 = 0.0, v1 = 0.0, v2 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+   if (valid && k < P.nd)
+   {
+      const double *vs = P.rm.vs + ((size_t)q * P.nd + k) * 3;
+      const double a0 = vs[0], a1 = vs[1], a2 = vs[2];
+      // covariant Piola v_i = sum_j vhat_j Jinv[j][i]
+      v0 = a0 * g.Ji[0] + a1 * g.Ji[3] + a2 * g.Ji[6];
+      v1 = a0 * g.Ji[1] + a1 * g.Ji[4] + a2 * g.Ji[7];
+      v2 = a0 * g.Ji[2] + a1 * g.Ji[5] + a2 * g.Ji[8];
+      if (rows == 6)
+      {
+         const double *cs = P.rm.cs + ((size_t)q * P.nd + k) * 3;
+         const double h0 = cs[0], h1 = cs[1], h2 = cs[2];
+         c0 = g.Jd[0] * h0 + g.Jd[1] * h1 + g.Jd[2] * h2;
+         c1 = g.Jd[3] * h0 + g.Jd[4] * h1 + g.Jd[5] * h2;
+         c2 = g.Jd[6] * h0 + g.Jd[7] * h1 + g.Jd[8] * h2;
+      }
+   }
+   const double2 wa = valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
+   const double2 wb = valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
+   const double2 al0 = make_double2(mc.m0r * v0 - mc.kxy * v1, mc.m0i * v0 + mc.c0y * c2);
+   const double2 al1 = make_double2(mc.m1r * v1 - mc.kxy * v0, mc.m1i * v1 - mc.c0x * c2);
+   const double2 al2 = make_double2(mc.m2r * v2, mc.m2i * v2 - mc.c0y * c0 + mc.c0x * c1);
+   const double2 C0 = cmul(wb, al0), C1 = cmul(wb, al1), C2 = cmul(wa, al2);
+   A[(row0 + 0) * NP + k] = v0;
+   A[(row0 + 1) * NP + k] = v1;
+   A[(row0 + 2) * NP + k] = v2;
+   Cr[(row0 + 0) * NP + k] = C0.x;
+   Ci[(row0 + 0) * NP + k] = C0.y;
+   Cr[(row0 + 1) * NP + k] = C1.x;
+   Ci[(row0 + 1) * NP + k] = C1.y;
+   Cr[(row0 + 2) * NP + k] = C2.x;
+   Ci[(row0 + 2) * NP + k] = C2.y;
+   if (rows == 6)
+   {
+      const double2 al3 = make_double2(0.0, mc.c1y * v2);
+      const double2 al4 = make_double2(0.0, -mc.c1x * v2);
+      const double2 al5 = make_double2(0.0, -mc.c1y * v0 + mc.c1x * v1);
+      const double2 C3 = cmul(wa, al3), C4 = cmul(wa, al4), C5 = cmul(wb, al5);
+      A[(row0 + 3) * NP + k] = c0;
+      A[(row0 + 4) * NP + k] = c1;
+      A[(row0 + 5) * NP + k] = c2;
+      Cr[(row0 + 3) * NP + k] = C3.x;
+      Ci[(row0 + 3) * NP + k] = C3.y;
+      Cr[(row0 + 4) * NP + k] = C4.x;
+      Ci[(row0 + 4) * NP + k] = C4.y;
+      Cr[(row0 + 5) * NP + k] = C5.x;
+      Ci[(row0 + 5) * NP + k] = C5.y;
+   }
+}
+
+// Form chunk `ch` of the combined K stream (cc chunks first, then mass
+// chunks) for the element group: items (point, basis) strided over nthr
+// threads starting at tid.  Staging: A, Cr, Ci of CHUNK_ROWS x NP each.
+__device__ __forceinline__ void form_chunk(const LocalParams &P, const ElemGeo &g,
+                                           const MassCoef &mc, const StreamShape &ss,
+                                           const double *qcoef, int ch, int tid, int nthr,
+                                           double *A, double *Cr, double *Ci, int NP)
+{
+   if (ch < ss.nch_cc)
+   {
+      const int nitem = ss.qpc_cc * NP;
+      for (int it = tid; it < nitem; it += nthr)
+      {
+         const int qi = it / NP, k = it - qi * NP;
+         const int q = ch * ss.qpc_cc + qi;
+         const bool valid = q < ss.nq_cc;
+         form_cc_item(P, g, qcoef + (size_t)(valid ? q : 0) * 4, valid ? q : 0, k, valid,
+                      A, Cr, Ci, qi * 3, NP);
+      }
+   }
+   else
+   {
+      const int c = ch - ss.nch_cc;
+      const int nitem = ss.qpc_m * NP;
+      for (int it = tid; it < nitem; it += nthr)
+      {
+         const int qi = it / NP, k = it - qi * NP;
+         const int q = c * ss.qpc_m + qi;
+         const bool valid = q < ss.nq_m;
+         const int qs = valid ? q : 0;
+         form_mass_item(P, g, mc, qcoef + (size_t)(P.rc.nq + qs) * 4, qs, k, valid,
+                        ss.rows_m, A, Cr, Ci, qi * ss.rows_m, NP);
+      }
+   }
+}
+
+// ---- MFMA local-matrix kernel ---------------------------------------------
+// v_mfma_f64_16x16x4_f64: A 16x4, B 4x16, D 16x16.  Lane l holds
+// A[l%16][l/16], B[l/16][l%16] and D[4(l/16)+v][l%16], v = 0..3.
+// Ae[k][l] = sum_K Bst[K][k] C[K][l]: A operand = staged shape rows,
+// B operand = staged coefficient rows (re and im separately).
+//
+// NT = column tiles (nd padded to 16 NT), RT = row tiles per wavefront,
+// WPE = NT / RT wavefronts per element, 4 / WPE elements per block.
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int NT, int RT>
+__global__ void __launch_bounds__(BLOCK) k_local_mfma(LocalParams P, int nqt)
+{
+   constexpr int NP = 16 * NT;
+   constexpr int WPE = NT / RT;
+   constexpr int EPB = 4 / WPE;
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int slot = wave / WPE, wsub = wave - slot * WPE;
+   const int e_raw = blockIdx.x * EPB + slot;
+   const bool live = e_raw < P.ne;
+   const int e = live ? e_raw : P.ne - 1;
+
+   extern __shared__ double smem[];
+   const int slot_doubles = nqt * 4 + 3 * CHUNK_ROWS * NP;
+   double *qcoef = smem + (size_t)slot * slot_doubles;
+   double *A = qcoef + nqt * 4;
+   double *Cr = A + CHUNK_ROWS * NP;
+   double *Ci = Cr + CHUNK_ROWS * NP;
+   const int gtid = wsub * 64 + lane;   // thread index within the element group
+
+   ElemGeo g;
+   load_geo(P, e, g);
+   for (int q = gtid; q < nqt; q += WPE * 64) { q_coef(P, g, q, qcoef + q * 4); }
+   const MassCoef mc = mass_coef(P, g.eps);
+   const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
+   __syncthreads();
+
+   dbl4 acc[RT][NT][2];
+#pragma unroll
+   for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+      {
+         acc[t][j][0] = dbl4{0.0, 0.0, 0.0, 0.0};
+         acc[t][j][1] = dbl4{0.0, 0.0, 0.0, 0.0};
+      }
+
+   const int g4 = lane >> 4, m16 = lane & 15;
+   const int nch = ss.nch_cc + ss.nch_m;
+   for (int ch = 0; ch < nch; ++ch)
+   {
+      form_chunk(P, g, mc, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
+      {
+         const int row = ks * 4 + g4;
+         double a[RT], br[NT], bi[NT];
+#pragma unroll
+         for (int t = 0; t < RT; ++t) { a[t] = A[row * NP + 16 * (wsub * RT + t) + m16]; }
+#pragma unroll
+         for (int j = 0; j < NT; ++j)
+         {
+            br[j] = Cr[row * NP + 16 * j + m16];
+            bi[j] = Ci[row * NP + 16 * j + m16];
+         }
+#pragma unroll
+         for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+            {
+               acc[t][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], br[j], acc[t][j][0], 0, 0, 0);
+               acc[t][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bi[j], acc[t][j][1], 0, 0, 0);
+            }
+      }
+      __syncthreads();
+   }
+
+   // dual face-DOF transform TInv(fo)^T A TInv: columns, then rows
+   // (solver/asm_gpu.inc:221-251)
+   const int nd = P.nd, p = P.p;
+   if (P.apply_te && p >= 2)
+   {
+      const int foff = 6 * p, ntd = p * (p - 1);
+      const int *fo = P.fo + (size_t)e * 4;
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+      {
+         const int col = 16 * j + m16;
+         double cs = 1.0, cp = 0.0;
+         if (col >= foff && col < foff + 4 * ntd)
+         {
+            const double *Ti = P.TInv + 4 * fo[(col - foff) / ntd];
+            const bool even = ((col - foff) & 1) == 0;
+            cs = even ? Ti[0] : Ti[3];
+            cp = even ? Ti[1] : Ti[2];
+         }
+#pragma unroll
+         for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+               for (int v = 0; v < 4; ++v)
+               {
+                  const double x = acc[t][j][c][v];
+                  const double y = __shfl_xor(x, 1);
+                  acc[t][j][c][v] = cs * x + cp * y;
+               }
+      }
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+      {
+#pragma unroll
+         for (int vp = 0; vp < 2; ++vp)
+         {
+            const int row = 16 * (wsub * RT + t) + 4 * g4 + 2 * vp;   // even row
+            if (row >= foff && row < foff + 4 * ntd)
+            {
+               const double *Ti = P.TInv + 4 * fo[(row - foff) / ntd];
+#pragma unroll
+               for (int j = 0; j < NT; ++j)
+#pragma unroll
+                  for (int c = 0; c < 2; ++c)
+                  {
+                     const double a0 = acc[t][j][c][2 * vp], b0 = acc[t][j][c][2 * vp + 1];
+                     acc[t][j][c][2 * vp] = Ti[0] * a0 + Ti[1] * b0;
+                     acc[t][j][c][2 * vp + 1] = Ti[2] * a0 + Ti[3] * b0;
+                  }
+            }
+         }
+      }
+   }
+
+   if (!live) { return; }
+   double2 *Ae = P.Ae + (size_t)e * nd * nd;
+#pragma unroll
+   for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+      {
+         const int col = 16 * j + m16;
+#pragma unroll
+         for (int v = 0; v < 4; ++v)
+         {
+            const int row = 16 * (wsub * RT + t) + 4 * g4 + v;
+            if (row < nd && col < nd)
+            {
+               Ae[(size_t)row * nd + col] = make_double2(acc[t][j][0][v], acc[t][j][1][v]);
+            }
+         }
+      }
+}
+
+// ---- generic VALU local-matrix kernel (any nd) -----------------------------
+// One block per element; the same K-stream staging; each thread accumulates
+// the entries idx = tid + BLOCK*i of the nd x nd matrix.  The face-DOF
+// transform runs afterwards in k_te.
+
+template <int NACC>
+__global__ void __launch_bounds__(BLOCK) k_local_generic(LocalParams P, int nqt, int NP)
+{
+   const int e = blockIdx.x;
+   const int tid = threadIdx.x;
+   extern __shared__ double smem[];
+   double *qcoef = smem;
+   double *A = qcoef + nqt * 4;
+   double *Cr = A + CHUNK_ROWS * NP;
+   double *Ci = Cr + CHUNK_ROWS * NP;
+
+   ElemGeo g;
+   load_geo(P, e, g);
+   for (int q = tid; q < nqt; q += BLOCK) { q_coef(P, g, q, qcoef + q * 4); }
+   const MassCoef mc = mass_coef(P, g.eps);
+   const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
+   __syncthreads();
+
+   const int nd = P.nd, n2 = nd * nd;
+   double ar[NACC], ai[NACC];
+#pragma unroll
+   for (int i = 0; i < NACC; ++i) { ar[i] = 0.0; ai[i] = 0.0; }
+
+   const int nch = ss.nch_cc + ss.nch_m;
+   for (int ch = 0; ch < nch; ++ch)
+   {
+      form_chunk(P, g, mc, ss, qcoef, ch, tid, BLOCK, A, Cr, Ci, NP);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < NACC; ++i)
+      {
+         const int idx = tid + BLOCK * i;
+         if (idx < n2)
+         {
+            const int k = idx / nd, l = idx - k * nd;
+            double sr = ar[i], si = ai[i];
+            for (int r = 0; r < CHUNK_ROWS; ++r)
+            {
+               const double a = A[r * NP + k];
+               sr += a * Cr[r * NP + l];
+               si += a * Ci[r * NP + l];
+            }
+            ar[i] = sr;
+            ai[i] = si;
+         }
+      }
+      __syncthreads();
+   }
+   double2 *Ae = P.Ae + (size_t)e * n2;
+#pragma unroll
+   for (int i = 0; i < NACC; ++i)
+   {
+      const int idx = tid + BLOCK * i;
+      if (idx < n2) { Ae[idx] = make_double2(ar[i], ai[i]); }
+   }
+}
+
+// Dual face-DOF transform in place on Ae (solver/asm_gpu.inc:221-251):
+// one block per element, columns then rows.
+__global__ void k_te(int nd, int p, const int *__restrict__ fo, const double *TInvG,
+                     double2 *__restrict__ Ae)
+{
+   const int e = blockIdx.x;
+   double2 *A = Ae + (size_t)e * nd * nd;
+   const int foff = 6 * p;
+   const int ntd = p * (p - 1);
+   for (int axis = 0; axis < 2; ++axis)
+   {
+      for (int f = 0; f < 4; ++f)
+      {
+         const double *Ti = TInvG + 4 * fo[(size_t)e * 4 + f];
+         for (int i = 0; i < ntd / 2; ++i)
+         {
+            const int d0 = foff + f * ntd + 2 * i;
+            for (int r = threadIdx.x; r < nd; r += blockDim.x)
+            {
+               double2 &a = axis == 0 ? A[(size_t)r * nd + d0] : A[(size_t)d0 * nd + r];
+               double2 &b = axis == 0 ? A[(size_t)r * nd + d0 + 1] : A[(size_t)(d0 + 1) * nd + r];
+               const double2 na = make_double2(Ti[0] * a.x + Ti[1] * b.x, Ti[0] * a.y + Ti[1] * b.y);
+               const double2 nb = make_double2(Ti[2] * a.x + Ti[3] * b.x, Ti[2] * a.y + Ti[3] * b.y);
+               a = na;
+               b = nb;
+            }
+         }
+      }
+      __syncthreads();
+   }
+}
+
+// ---- CSR slot map ----------------------------------------------------------
+// slot[(e*nd + a)*nd + b] = position of column dof[e][b] inside row
+// dof[e][a] (lower bound in the sorted row, as solver/asm_gpu.inc:447-453
+// and solver/asm_data.inc:393).  A missing column sets *bad.
+__global__ void k_slot_map(int ne, int nd, const int *__restrict__ dof,
+                           const int *__restrict__ Ip, const int *__restrict__ Jc,
+                           uint16_t *__restrict__ slot, int *__restrict__ bad)
+{
+   const size_t n2 = (size_t)nd * nd;
+   const size_t total = (size_t)ne * n2;
+   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+        idx += (size_t)gridDim.x * blockDim.x)
+   {
+      const size_t e = idx / n2;
+      const int ab = (int)(idx - e * n2);
+      const int a = ab / nd, b = ab - a * nd;
+      const int r = dof[e * nd + a], c = dof[e * nd + b];
+      int lo = Ip[r], hi = Ip[r + 1];
+      const int base = lo;
+      while (lo < hi)
+      {
+         const int mid = (lo + hi) >> 1;
+         if (Jc[mid] < c) { lo = mid + 1; }
+         else { hi = mid; }
+      }
+      if (lo >= Ip[r + 1] || Jc[lo] != c) { *bad = 1; slot[idx] = 0; }
+      else { slot[idx] = (uint16_t)(lo - base); }
+   }
+}
+
+// ---- deterministic row-segmented scatter + elimination ----------------------
+// One wavefront per CSR row r.  The row is accumulated in an LDS buffer from
+// its contributors (element, local index) in ascending element order --
+// the summation order of the reference's CPU reassembler
+// (solver/asm_data.inc:381-400) -- then written once.  With `eliminate`,
+// essential rows become (0 ... 1 ... 0) and essential columns 0
+// (solver/asm_gpu.inc:463-478).
+struct ScatterParams {
+   int n, nd;
+   const int *Ip, *Jc;
+   const int *dof;
+   const double *sign;
+   const int *rc_ptr, *rc_list;
+   const uint16_t *slot;
+   const unsigned char *is_ess;
+   const double2 *Ae;
+   double2 *vals;
+   int eliminate;
+   int maxrow;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
+{
+   extern __shared__ double2 rowbuf[];
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + wave;
+   if (r >= S.n) { return; }
+   double2 *buf = rowbuf + (size_t)wave * S.maxrow;
+   const int base = S.Ip[r];
+   const int len = S.Ip[r + 1] - base;
+   for (int s = lane; s < len; s += 64) { buf[s] = make_double2(0.0, 0.0); }
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+   const bool row_ess = S.eliminate && S.is_ess[r];
+   const int nd = S.nd;
+   if (!row_ess)
+   {
+      const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
+      for (int c = c0; c < c1; ++c)
+      {
+         const int ea = S.rc_list[c];          // e*nd + a
+         const int e = ea / nd;
+         const double sa = S.sign[ea];
+         for (int b = lane; b < nd; b += 64)
+         {
+            const int eb = e * nd + b;
+            if (S.eliminate && S.is_ess[S.dof[eb]]) { continue; }
+            const int s = S.slot[(size_t)ea * nd + b];
+            const double sg = sa * S.sign[eb];
+            const double2 v = S.Ae[(size_t)ea * nd + b];
+            double2 t = buf[s];
+            t.x += sg * v.x;
+            t.y += sg * v.y;
+            buf[s] = t;
+         }
+         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+         __builtin_amdgcn_wave_barrier();
+         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+   }
+   else
+   {
+      for (int s = lane; s < len; s += 64)
+      {
+         if (S.Jc[base + s] == r) { buf[s] = make_double2(1.0, 0.0); }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+   }
+   for (int s = lane; s < len; s += 64) { S.vals[base + s] = buf[s]; }
+}
+
+// ---- complex CSR SpMV (solver/asm_gpu.inc:481-493) ------------------------
+__global__ void k_spmv(int n, const int *__restrict__ Ip, const int *__restrict__ Jc,
+                       const double2 *__restrict__ vals, const double2 *__restrict__ x,
+                       double2 *__restrict__ y)
+{
+   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= n) { return; }
+   double yr = 0.0, yi = 0.0;
+   for (int k = Ip[r]; k < Ip[r + 1]; ++k)
+   {
+      const double2 a = vals[k], b = x[Jc[k]];
+      yr += a.x * b.x - a.y * b.y;
+      yi += a.x * b.y + a.y * b.x;
+   }
+   y[r] = make_double2(yr, yi);
+}
+
+__global__ void k_fill_zero(double2 *p, size_t n)
+{
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+        i += (size_t)gridDim.x * blockDim.x)
+   {
+      p[i] = make_double2(0.0, 0.0);
+   }
+}
+
+} // namespace lfasm

@@ -1,0 +1,111 @@
+/*
+ * CPU oracle for the Nedelec system-matrix assembly path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's CPU reassembler (solver/asm_data.inc:178-402) and of the
+ * coefficient helpers it calls (solver/problem_common.inc:69-98,150-158),
+ * used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+ * leg as the checker.  Nothing in the lithofem_amd product links or calls it.
+ *
+ * Parity status of each function is recorded next to its declaration:
+ * "pinned" means tests/ check it against reference-held tables, closed forms,
+ * invariants or brute force; nothing here is "parity unpinned" except where
+ * stated.
+ */
+#ifndef LITHOFEM_ASM_ORACLE_H
+#define LITHOFEM_ASM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* integrator selection bits (solver/asm_gpu.inc:51-58) */
+enum { ORC_CURLCURL = 1, ORC_MASS = 2, ORC_CROSS0 = 4, ORC_CROSS1 = 8, ORC_KLK = 16 };
+
+typedef struct {
+   int nq;
+   const double *ip;     /* nq*3 */
+   const double *w;      /* nq */
+   const double *vshape; /* nq*nd*3, [(q*nd+k)*3+d] */
+   const double *curl;   /* nq*nd*3 */
+} orc_rule;
+
+/* Problem subset the assembly reads (solver/problem_common.inc:18-65). */
+typedef struct {
+   double k0, kx, ky;
+   double z_min, z_max;
+   double pml_thick, pml_order, pml_target;
+   int nreg;
+   const double *eps;        /* nreg interleaved complex, by attribute-1 */
+   const int *is_pml;        /* nreg */
+   const int *pml_top;       /* nreg */
+} orc_problem;
+
+/* AsmData equivalent (solver/asm_data.inc:77-97). */
+typedef struct {
+   int ne, nd, p, height, has_kpar;
+   const int *dof;           /* ne*nd decoded */
+   const double *sign;       /* ne*nd */
+   const int *fo;            /* ne*4 (p >= 2) */
+   const int *attr;          /* ne, 1-based */
+   const double *J;          /* ne*9 row-major */
+   const double *Jinv;       /* ne*9 row-major */
+   const double *detJ;       /* ne */
+   const double *v0;         /* ne*3 */
+   orc_rule rule_cc, rule_mass;
+   const int *I;             /* height+1 */
+   const int *Jcol;          /* nnz */
+   const double *eps_elem;   /* optional ne interleaved complex; NULL = region eps */
+} orc_asm_data;
+
+/* solver/problem_common.inc:69-90 (pinned: closed-form values) */
+void orc_s_z(const orc_problem *pb, double z, double out[2]);
+
+/* solver/asm_data.inc:218-370 with an integrator mask and a TE switch;
+ * Ae = nd*nd interleaved complex, row k, column l.  (pinned) */
+void orc_reassemble_element(const orc_asm_data *ad, const orc_problem *pb,
+                            int e, int mask, int apply_te, double *Ae);
+
+/* elements [e0, e1) -> Ae_out (element-major), for timing samples */
+void orc_reassemble_range(const orc_asm_data *ad, const orc_problem *pb,
+                          int e0, int e1, int mask, int apply_te, double *Ae_out);
+
+/* solver/asm_data.inc:374-402: full reassembly into the symbolic CSR,
+ * split re/im values aligned with (I, Jcol).  Returns 0 or -1 if a CSR
+ * position is missing.  (pinned) */
+int orc_reassemble_all(const orc_asm_data *ad, const orc_problem *pb,
+                       double *vre, double *vim);
+
+/* scatter of precomputed local matrices (same order as orc_reassemble_all) */
+int orc_scatter(const orc_asm_data *ad, const double *Ae_all,
+                double *vre, double *vim);
+
+/* solver/asm_data.inc:178-210: I (height+1) filled, *Jcol_out malloc'ed.
+ * Returns nnz, or -1.  (pinned: brute force on small meshes) */
+int64_t orc_build_csr_symbolic(int ne, int nd, int height, const int *dof,
+                               int *I, int **Jcol_out);
+void orc_free(void *p);
+
+/* solver/asm_gpu.inc:463-478: zero essential rows/cols, unit diagonal. */
+void orc_eliminate(int n, const int *I, const int *Jcol,
+                   const unsigned char *is_ess, double *vre, double *vim);
+
+/* solver/asm_gpu.inc:481-493,533-565: rel = ||A X - B|| / ||B|| with X, B
+ * as 2n [re; im] vectors. */
+double orc_residual(int n, const int *I, const int *Jcol, const double *vre,
+                    const double *vim, const double *X2n, const double *B2n);
+
+/* Per-element adjoint sensitivity (no reference counterpart; pinned by
+ * central finite differences of the assembled matrix): for each element e,
+ * g[e] = lam_e^T (dA_e/d eps_e) u_e, complex, where dA_e/d eps_e is the mass
+ * term with eps = 1 and the local vectors are gathered from global complex
+ * vectors with signs and the face-pair transform. */
+void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
+                         const double *lam, const double *u, double *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

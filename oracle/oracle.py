@@ -1,0 +1,199 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+to check or time against; the lithofem_amd product never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liborc.so"
+
+CURLCURL, MASS, CROSS0, CROSS1, KLK = 1, 2, 4, 8, 16
+ALL = CURLCURL | MASS | CROSS0 | CROSS1 | KLK
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+class _Rule(ct.Structure):
+    _fields_ = [("nq", ct.c_int), ("ip", ct.c_void_p), ("w", ct.c_void_p),
+                ("vshape", ct.c_void_p), ("curl", ct.c_void_p)]
+
+
+class _Problem(ct.Structure):
+    _fields_ = [("k0", ct.c_double), ("kx", ct.c_double), ("ky", ct.c_double),
+                ("z_min", ct.c_double), ("z_max", ct.c_double),
+                ("pml_thick", ct.c_double), ("pml_order", ct.c_double),
+                ("pml_target", ct.c_double), ("nreg", ct.c_int),
+                ("eps", ct.c_void_p), ("is_pml", ct.c_void_p), ("pml_top", ct.c_void_p)]
+
+
+class _AsmData(ct.Structure):
+    _fields_ = [("ne", ct.c_int), ("nd", ct.c_int), ("p", ct.c_int), ("height", ct.c_int),
+                ("has_kpar", ct.c_int), ("dof", ct.c_void_p), ("sign", ct.c_void_p),
+                ("fo", ct.c_void_p), ("attr", ct.c_void_p), ("J", ct.c_void_p),
+                ("Jinv", ct.c_void_p), ("detJ", ct.c_void_p), ("v0", ct.c_void_p),
+                ("rule_cc", _Rule), ("rule_mass", _Rule), ("I", ct.c_void_p),
+                ("Jcol", ct.c_void_p), ("eps_elem", ct.c_void_p)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ct.CDLL(str(LIB))
+        L.orc_s_z.argtypes = [ct.c_void_p, ct.c_double, ct.c_void_p]
+        L.orc_reassemble_element.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                             ct.c_int, ct.c_void_p]
+        L.orc_reassemble_range.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                           ct.c_int, ct.c_int, ct.c_void_p]
+        L.orc_reassemble_all.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_reassemble_all.restype = ct.c_int
+        L.orc_scatter.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_scatter.restype = ct.c_int
+        L.orc_build_csr_symbolic.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_void_p,
+                                             ct.c_void_p, ct.POINTER(ct.c_void_p)]
+        L.orc_build_csr_symbolic.restype = ct.c_int64
+        L.orc_free.argtypes = [ct.c_void_p]
+        L.orc_eliminate.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                    ct.c_void_p, ct.c_void_p]
+        L.orc_residual.argtypes = [ct.c_int] + [ct.c_void_p] * 6
+        L.orc_residual.restype = ct.c_double
+        L.orc_eps_sensitivity.argtypes = [ct.c_void_p] * 5
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """Holds the ctypes views of one (AsmData, Problem) pair."""
+
+    def __init__(self, ad, pb, I=None, Jcol=None, eps_elem=None):
+        self.ad, self.pb = ad, pb
+        self._keep = []
+
+        def keep(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        self.prob = _Problem(pb.k0, pb.kx, pb.ky, pb.z_min, pb.z_max, pb.pml_thick,
+                             pb.pml_order, pb.pml_target, len(pb.regions),
+                             _ptr(keep(pb.eps_tab(), np.float64)),
+                             _ptr(keep(pb.is_pml(), np.int32)),
+                             _ptr(keep(pb.pml_top(), np.int32)))
+
+        def rule(r):
+            return _Rule(r.nq, _ptr(keep(r.ip, np.float64)), _ptr(keep(r.w, np.float64)),
+                         _ptr(keep(r.vshape, np.float64)), _ptr(keep(r.curl, np.float64)))
+
+        self.I = keep(I, np.int32)
+        self.Jcol = keep(Jcol, np.int32)
+        ee = None
+        if eps_elem is not None:
+            ee = keep(np.asarray(eps_elem, dtype=np.complex128).view(np.float64), np.float64)
+        fo = ad.fo if ad.p >= 2 else np.zeros(4 * ad.ne, np.int32)
+        self.data = _AsmData(ad.ne, ad.nd, ad.p, ad.height, 1 if pb.has_kpar else 0,
+                             _ptr(keep(ad.dof, np.int32)), _ptr(keep(ad.sign, np.float64)),
+                             _ptr(keep(fo, np.int32)), _ptr(keep(ad.attr, np.int32)),
+                             _ptr(keep(ad.J, np.float64)), _ptr(keep(ad.Jinv, np.float64)),
+                             _ptr(keep(ad.detJ, np.float64)), _ptr(keep(ad.v0, np.float64)),
+                             rule(ad.rule_cc), rule(ad.rule_mass),
+                             _ptr(self.I), _ptr(self.Jcol), _ptr(ee))
+
+    def s_z(self, z: float) -> complex:
+        out = np.zeros(2)
+        lib().orc_s_z(ct.byref(self.prob), z, _ptr(out))
+        return complex(out[0], out[1])
+
+    def element(self, e: int, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
+        nd = self.ad.nd
+        out = np.zeros((nd, nd), dtype=np.complex128)
+        lib().orc_reassemble_element(ct.byref(self.data), ct.byref(self.prob), e, mask,
+                                     1 if apply_te else 0, _ptr(out))
+        return out
+
+    def elements(self, e0: int, e1: int, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
+        nd = self.ad.nd
+        out = np.zeros((e1 - e0, nd, nd), dtype=np.complex128)
+        lib().orc_reassemble_range(ct.byref(self.data), ct.byref(self.prob), e0, e1, mask,
+                                   1 if apply_te else 0, _ptr(out))
+        return out
+
+    def assemble(self):
+        nnz = int(self.I[-1])
+        vre = np.zeros(nnz)
+        vim = np.zeros(nnz)
+        rc = lib().orc_reassemble_all(ct.byref(self.data), ct.byref(self.prob),
+                                      _ptr(vre), _ptr(vim))
+        if rc != 0:
+            raise RuntimeError("CSR position missing")
+        return vre, vim
+
+    def scatter(self, Ae_all: np.ndarray):
+        nnz = int(self.I[-1])
+        vre = np.zeros(nnz)
+        vim = np.zeros(nnz)
+        Ae_all = np.ascontiguousarray(Ae_all, dtype=np.complex128)
+        rc = lib().orc_scatter(ct.byref(self.data), _ptr(Ae_all), _ptr(vre), _ptr(vim))
+        if rc != 0:
+            raise RuntimeError("CSR position missing")
+        return vre, vim
+
+    def eps_sensitivity(self, lam: np.ndarray, u: np.ndarray) -> np.ndarray:
+        lam = np.ascontiguousarray(lam, dtype=np.complex128)
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        g = np.zeros(self.ad.ne, dtype=np.complex128)
+        lib().orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob),
+                                  _ptr(lam), _ptr(u), _ptr(g))
+        return g
+
+
+def csr_symbolic(dof: np.ndarray, height: int):
+    ne, nd = dof.shape
+    dof = np.ascontiguousarray(dof, dtype=np.int32)
+    I = np.zeros(height + 1, dtype=np.int32)
+    pj = ct.c_void_p()
+    nnz = lib().orc_build_csr_symbolic(ne, nd, height, _ptr(dof), _ptr(I), ct.byref(pj))
+    if nnz < 0:
+        raise MemoryError("csr symbolic")
+    Jc = np.ctypeslib.as_array(ct.cast(pj, ct.POINTER(ct.c_int32)), shape=(max(nnz, 1),))
+    Jc = Jc[:nnz].copy()
+    lib().orc_free(pj)
+    return I, Jc
+
+
+def eliminate(I, Jcol, is_ess, vre, vim):
+    vre = np.array(vre, dtype=np.float64)
+    vim = np.array(vim, dtype=np.float64)
+    I = np.ascontiguousarray(I, dtype=np.int32)
+    Jcol = np.ascontiguousarray(Jcol, dtype=np.int32)
+    ess = np.ascontiguousarray(is_ess, dtype=np.uint8)
+    lib().orc_eliminate(len(I) - 1, _ptr(I), _ptr(Jcol), _ptr(ess), _ptr(vre), _ptr(vim))
+    return vre, vim
+
+
+def residual(I, Jcol, vre, vim, X2n, B2n) -> float:
+    arrs = [np.ascontiguousarray(a, dtype=dt) for a, dt in
+            ((I, np.int32), (Jcol, np.int32), (vre, np.float64), (vim, np.float64),
+             (X2n, np.float64), (B2n, np.float64))]
+    return lib().orc_residual(len(arrs[0]) - 1, *[_ptr(a) for a in arrs])
