@@ -122,6 +122,13 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
  * then (apply_te, p >= 2) the dual face-DOF transform.  Ae is overwritten. */
 int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te);
 
+/* lfasm_compute_local with an explicit kernel choice (0 = automatic:
+ * FP64 MFMA tiles for nd >= 45, VALU otherwise; 1 = VALU kernel;
+ * 2 = MFMA kernel).  No reference counterpart: used by the parity tests to
+ * check both kernels and by the launch autotuner. */
+int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te,
+                               int kernel);
+
 /* AsmGpuGlobal::ScatterEliminate (solver/asm_gpu.inc:517-530): CSR values
  * from the local matrices by a deterministic per-row segmented reduction
  * (contributions summed in ascending element order), then (eliminate)

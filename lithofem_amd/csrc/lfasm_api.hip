@@ -1,0 +1,707 @@
+// Host side of the gfx950 Nedelec assembly: the C-ABI of include/lithofem_asm.h.
+//
+// One lfasm_handle replaces one AsmGpuGlobal object of the reference
+// (solver/asm_gpu.inc:271-599): it owns every device buffer of one mesh and
+// one HIP stream on its device.  Errors are returned as status codes with a
+// message string instead of printf + bool (solver/asm_gpu.inc:255-263).
+#include "lfasm_device.hpp"
+#include "../../include/lithofem_asm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace lfasm;
+
+namespace {
+
+struct DevRule {
+   int nq = 0;
+   double *ip = nullptr, *w = nullptr, *vs = nullptr, *cs = nullptr;
+};
+
+enum { T_LOCAL = 0, T_SCATTER = 1, T_BUILD = 2, T_BATCH = 3, T_ADJ = 4, T_N = 5 };
+
+}  // namespace
+
+struct lfasm_handle {
+   int device = 0;
+   hipStream_t stream = nullptr;
+   std::string err;
+   // mesh
+   int ne = 0, nd = 0, p = 0, height = 0, nreg = 0;
+   bool inited = false, global_built = false, local_valid = false;
+   double *J = nullptr, *Ji = nullptr, *detJ = nullptr, *v0 = nullptr;
+   int *attr = nullptr, *fo = nullptr, *dof = nullptr;
+   double *sign = nullptr;
+   double2 *eps_tab = nullptr, *eps_elem = nullptr;
+   DevRule rc, rm;
+   PmlDev pml{};
+   double TInv[24];
+   double *TInv_dev = nullptr;
+   double2 *Ae = nullptr;
+   // global
+   int64_t nnz = 0;
+   int maxrow = 0;
+   int *Ip = nullptr, *Jc = nullptr, *rc_ptr = nullptr, *rc_list = nullptr, *bad = nullptr;
+   uint16_t *slot = nullptr;
+   unsigned char *ess = nullptr;
+   double2 *vals = nullptr;
+   // timing
+   hipEvent_t ev[2 * T_N] = {};
+   bool ev_used[T_N] = {};
+   int mfma_variant = 0;
+
+   int fail(int code, const char *fmt, ...)
+   {
+      char buf[512];
+      va_list ap;
+      va_start(ap, fmt);
+      vsnprintf(buf, sizeof(buf), fmt, ap);
+      va_end(ap);
+      err = buf;
+      return code;
+   }
+};
+
+#define LF_HIP(call)                                                             \
+   do {                                                                          \
+      hipError_t e_ = (call);                                                    \
+      if (e_ != hipSuccess) {                                                    \
+         return h->fail(LFASM_E_HIP, "HIP error '%s' at %s:%d (%s)",             \
+                        hipGetErrorString(e_), __FILE__, __LINE__, #call);       \
+      }                                                                          \
+   } while (0)
+
+namespace {
+
+template <typename T>
+hipError_t upload(lfasm_handle *h, T *&dst, const T *src, size_t n)
+{
+   if (dst) { (void)hipFree(dst); dst = nullptr; }
+   hipError_t e = hipMalloc((void **)&dst, std::max<size_t>(n, 1) * sizeof(T));
+   if (e != hipSuccess) { return e; }
+   if (n == 0) { return hipSuccess; }
+   e = hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream);
+   if (e != hipSuccess) { return e; }
+   return hipStreamSynchronize(h->stream);
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+   if (p) { (void)hipFree((void *)p); p = nullptr; }
+}
+
+void release_mesh(lfasm_handle *h)
+{
+   dfree(h->J); dfree(h->Ji); dfree(h->detJ); dfree(h->v0);
+   dfree(h->attr); dfree(h->fo); dfree(h->dof); dfree(h->sign);
+   dfree(h->eps_tab); dfree(h->eps_elem); dfree(h->TInv_dev); dfree(h->Ae);
+   for (DevRule *r : {&h->rc, &h->rm})
+   {
+      dfree(r->ip); dfree(r->w); dfree(r->vs); dfree(r->cs);
+      r->nq = 0;
+   }
+   h->inited = false;
+   h->local_valid = false;
+}
+
+void release_global(lfasm_handle *h)
+{
+   dfree(h->Ip); dfree(h->Jc); dfree(h->rc_ptr); dfree(h->rc_list);
+   dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->vals);
+   h->global_built = false;
+}
+
+int check_rule(lfasm_handle *h, const lfasm_rule &r, const char *name)
+{
+   if (r.nq <= 0 || !r.ip || !r.w || !r.vshape || !r.curl)
+   {
+      return h->fail(LFASM_E_ARG, "rule %s: nq must be > 0 and all tables set", name);
+   }
+   return LFASM_OK;
+}
+
+hipError_t upload_rule(lfasm_handle *h, DevRule &d, const lfasm_rule &r, int nd)
+{
+   d.nq = r.nq;
+   hipError_t e;
+   if ((e = upload(h, d.ip, r.ip, (size_t)r.nq * 3)) != hipSuccess) { return e; }
+   if ((e = upload(h, d.w, r.w, (size_t)r.nq)) != hipSuccess) { return e; }
+   if ((e = upload(h, d.vs, r.vshape, (size_t)r.nq * nd * 3)) != hipSuccess) { return e; }
+   return upload(h, d.cs, r.curl, (size_t)r.nq * nd * 3);
+}
+
+LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te,
+                         double2 *Ae)
+{
+   LocalParams P;
+   P.ne = h->ne;
+   P.nd = h->nd;
+   P.p = h->p;
+   P.J = h->J;
+   P.Ji = h->Ji;
+   P.detJ = h->detJ;
+   P.v0 = h->v0;
+   P.attr = h->attr;
+   P.eps_tab = h->eps_tab;
+   P.eps_elem = h->eps_elem;
+   P.fo = h->fo;
+   P.rc = RuleDev{h->rc.nq, h->rc.ip, h->rc.w, h->rc.vs, h->rc.cs};
+   P.rm = RuleDev{h->rm.nq, h->rm.ip, h->rm.w, h->rm.vs, h->rm.cs};
+   P.pml = h->pml;
+   std::memcpy(P.TInv, h->TInv, sizeof(P.TInv));
+   P.use_cc = (mask_cc & LFASM_CURLCURL) ? 1 : 0;
+   int mf = 0;
+   if (mask_mass & LFASM_MASS) { mf |= 1; }
+   const bool kpar = h->pml.kx != 0.0 || h->pml.ky != 0.0;
+   if (kpar)
+   {
+      // the k-parallel integrators exist only when k_par != 0
+      // (solver/lithofem_solve.cpp:695-704)
+      if (mask_mass & LFASM_CROSS0) { mf |= 2; }
+      if (mask_mass & LFASM_CROSS1) { mf |= 4; }
+      if (mask_mass & LFASM_KLK) { mf |= 8; }
+   }
+   P.mflags = mf;
+   P.apply_te = apply_te && h->p >= 2;
+   P.Ae = Ae;
+   return P;
+}
+
+template <int NT, int RT, int EPB, int WPEU>
+hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P)
+{
+   constexpr int NP = 16 * NT;
+   constexpr int WPE = NT / RT;
+   constexpr int THREADS = 64 * WPE * EPB;
+   const int nqt = P.rc.nq + P.rm.nq;
+   const size_t shmem = (size_t)EPB * (EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP) * sizeof(double);
+   hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma<NT, RT, EPB, WPEU>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   if (e != hipSuccess) { return e; }
+   const int grid = (P.ne + EPB - 1) / EPB;
+   hipLaunchKernelGGL((k_local_mfma<NT, RT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
+                      h->stream, P, nqt);
+   return hipGetLastError();
+}
+
+template <int NACC, int NTHR>
+hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
+{
+   const int nqt = P.rc.nq + P.rm.nq;
+   const int NP = P.nd;
+   const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP) * sizeof(double);
+   hipError_t e = hipFuncSetAttribute((const void *)k_local_generic<NACC, NTHR>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   if (e != hipSuccess) { return e; }
+   hipLaunchKernelGGL((k_local_generic<NACC, NTHR>), dim3(P.ne), dim3(NTHR), shmem, h->stream,
+                      P, nqt, NP);
+   e = hipGetLastError();
+   if (e != hipSuccess || !P.apply_te) { return e; }
+   hipLaunchKernelGGL(k_te, dim3(P.ne), dim3(64), 0, h->stream, P.nd, P.p, P.fo, h->TInv_dev,
+                      P.Ae);
+   return hipGetLastError();
+}
+
+// MFMA launch variants per tile count: (row tiles per wavefront, elements
+// per block, waves per EU).  Variant 0 of each list is the default; the
+// launch autotuner (lithofem_amd.tune) times all of them.
+constexpr int N_MFMA_VARIANTS = 4;
+
+hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, int v)
+{
+   switch (nt * 16 + v)
+   {
+   case 1 * 16 + 0: case 1 * 16 + 1: case 1 * 16 + 2: case 1 * 16 + 3:
+      return launch_mfma<1, 1, 4, 2>(h, P);
+   case 2 * 16 + 0: case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P);
+   case 2 * 16 + 1: case 2 * 16 + 3: return launch_mfma<2, 2, 4, 2>(h, P);
+   case 3 * 16 + 0: return launch_mfma<3, 1, 1, 2>(h, P);
+   case 3 * 16 + 1: return launch_mfma<3, 3, 4, 1>(h, P);
+   case 3 * 16 + 2: return launch_mfma<3, 1, 1, 3>(h, P);
+   case 3 * 16 + 3: return launch_mfma<3, 3, 2, 1>(h, P);
+   case 6 * 16 + 0: case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
+   case 6 * 16 + 1: case 6 * 16 + 3: return launch_mfma<6, 2, 1, 1>(h, P);
+   default: return hipErrorInvalidValue;
+   }
+}
+
+// Kernel choice: `kernel` = 0 automatic (MFMA for nd >= 45 with the
+// handle's tuned variant, VALU otherwise), 1 = VALU kernel,
+// 2 + v = MFMA variant v.
+hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel)
+{
+   const int nd = P.nd;
+   const int nt = nd <= 16 ? 1 : nd <= 32 ? 2 : nd <= 48 ? 3 : 6;
+   if (kernel >= 2 && kernel < 2 + N_MFMA_VARIANTS) { return launch_mfma_variant(h, P, nt, kernel - 2); }
+   if (kernel == 0 && nd >= 45) { return launch_mfma_variant(h, P, nt, h->mfma_variant); }
+   const int n2 = nd * nd;
+   if (n2 <= 256) { return launch_generic<1, 256>(h, P); }
+   if (n2 <= 512) { return launch_generic<2, 256>(h, P); }
+   if (n2 <= 2048) { return launch_generic<4, 512>(h, P); }
+   if (n2 <= 8192) { return launch_generic<8, 1024>(h, P); }
+   return hipErrorInvalidValue;
+}
+
+hipError_t ev_begin(lfasm_handle *h, int t)
+{
+   h->ev_used[t] = true;
+   return hipEventRecord(h->ev[2 * t], h->stream);
+}
+
+hipError_t ev_end(lfasm_handle *h, int t) { return hipEventRecord(h->ev[2 * t + 1], h->stream); }
+
+}  // namespace
+
+// ---- C-ABI -------------------------------------------------------------------
+
+extern "C" {
+
+int lfasm_create(int device, lfasm_handle **out)
+{
+   if (!out) { return LFASM_E_ARG; }
+   *out = nullptr;
+   lfasm_handle *h = new lfasm_handle();
+   h->device = device;
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+   {
+      *out = h;
+      return h->fail(LFASM_E_HIP, "no HIP device");
+   }
+   if (device < 0 || device >= ndev)
+   {
+      *out = h;
+      return h->fail(LFASM_E_ARG, "device %d out of range (%d devices)", device, ndev);
+   }
+   *out = h;
+   LF_HIP(hipSetDevice(device));
+   LF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+   for (int i = 0; i < 2 * T_N; ++i) { LF_HIP(hipEventCreate(&h->ev[i])); }
+   return LFASM_OK;
+}
+
+void lfasm_destroy(lfasm_handle *h)
+{
+   if (!h) { return; }
+   (void)hipSetDevice(h->device);
+   if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+   release_global(h);
+   release_mesh(h);
+   for (int i = 0; i < 2 * T_N; ++i)
+   {
+      if (h->ev[i]) { (void)hipEventDestroy(h->ev[i]); }
+   }
+   if (h->stream) { (void)hipStreamDestroy(h->stream); }
+   delete h;
+}
+
+const char *lfasm_last_error(const lfasm_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+double lfasm_pml_smax(double target, double order, double k0, double n_med, double thick)
+{
+   return -std::log(target) * (order + 1) / (2.0 * k0 * n_med * thick);
+}
+
+int lfasm_init(lfasm_handle *h, const lfasm_mesh *m)
+{
+   if (!h || !h->stream) { return LFASM_E_STATE; }
+   if (std::getenv("LITHOFEM_ASM_FORCE_FAIL"))
+   {
+      return h->fail(LFASM_E_FORCED, "forced failure injected (LITHOFEM_ASM_FORCE_FAIL)");
+   }
+   if (!m) { return h->fail(LFASM_E_ARG, "null mesh"); }
+   if (m->ne <= 0 || m->nd <= 0 || m->p <= 0 || m->height <= 0 || m->nreg <= 0)
+   {
+      return h->fail(LFASM_E_ARG, "ne, nd, p, height, nreg must be positive");
+   }
+   if (m->p > 4 || m->nd != m->p * (m->p + 2) * (m->p + 3) / 2)
+   {
+      return h->fail(LFASM_E_UNSUPPORTED, "order %d / nd %d outside the kernels (p <= 4)",
+                     m->p, m->nd);
+   }
+   if (!m->dof || !m->sign || !m->attr || !m->J || !m->Jinv || !m->detJ || !m->v0 ||
+       !m->eps_tab || !m->TInv || (m->p >= 2 && !m->fo))
+   {
+      return h->fail(LFASM_E_ARG, "missing mesh array");
+   }
+   int rc;
+   if ((rc = check_rule(h, m->rule_cc, "cc")) != LFASM_OK) { return rc; }
+   if ((rc = check_rule(h, m->rule_mass, "mass")) != LFASM_OK) { return rc; }
+   // host-side validation of everything the kernels index with
+   for (size_t i = 0; i < (size_t)m->ne * m->nd; ++i)
+   {
+      if (m->dof[i] < 0 || m->dof[i] >= m->height)
+      {
+         return h->fail(LFASM_E_ARG, "dof[%zu] = %d outside [0, %d)", i, m->dof[i], m->height);
+      }
+   }
+   for (int e = 0; e < m->ne; ++e)
+   {
+      if (m->attr[e] < 1 || m->attr[e] > m->nreg)
+      {
+         return h->fail(LFASM_E_ARG, "attr[%d] = %d outside [1, %d]", e, m->attr[e], m->nreg);
+      }
+      if (!(m->detJ[e] > 0.0))
+      {
+         return h->fail(LFASM_E_ARG, "detJ[%d] <= 0 (expected positively oriented tet)", e);
+      }
+   }
+   if (m->p >= 2)
+   {
+      for (size_t i = 0; i < (size_t)m->ne * 4; ++i)
+      {
+         if (m->fo[i] < 0 || m->fo[i] > 5)
+         {
+            return h->fail(LFASM_E_ARG, "fo[%zu] = %d outside 0..5", i, m->fo[i]);
+         }
+      }
+   }
+   LF_HIP(hipSetDevice(h->device));
+   release_global(h);
+   release_mesh(h);
+   h->ne = m->ne;
+   h->nd = m->nd;
+   h->p = m->p;
+   h->height = m->height;
+   h->nreg = m->nreg;
+   const size_t ne = (size_t)m->ne, nd = (size_t)m->nd;
+   LF_HIP(upload(h, h->J, m->J, ne * 9));
+   LF_HIP(upload(h, h->Ji, m->Jinv, ne * 9));
+   LF_HIP(upload(h, h->detJ, m->detJ, ne));
+   LF_HIP(upload(h, h->v0, m->v0, ne * 3));
+   LF_HIP(upload(h, h->attr, m->attr, ne));
+   LF_HIP(upload(h, h->dof, m->dof, ne * nd));
+   LF_HIP(upload(h, h->sign, m->sign, ne * nd));
+   if (m->p >= 2) { LF_HIP(upload(h, h->fo, m->fo, ne * 4)); }
+   LF_HIP(upload_rule(h, h->rc, m->rule_cc, m->nd));
+   LF_HIP(upload_rule(h, h->rm, m->rule_mass, m->nd));
+   LF_HIP(upload(h, h->eps_tab, (const double2 *)m->eps_tab, (size_t)m->nreg));
+   if (m->eps_elem) { LF_HIP(upload(h, h->eps_elem, (const double2 *)m->eps_elem, ne)); }
+   std::memcpy(h->TInv, m->TInv, sizeof(h->TInv));
+   LF_HIP(upload(h, h->TInv_dev, h->TInv, 24));
+   h->pml = PmlDev{m->pml.z_min, m->pml.z_max, m->pml.thick, m->pml.order,
+                   m->pml.smax_bot, m->pml.smax_top, m->pml.k0, m->pml.kx, m->pml.ky};
+   LF_HIP(hipMalloc((void **)&h->Ae, ne * nd * nd * sizeof(double2)));
+   h->inited = true;
+   return LFASM_OK;
+}
+
+int lfasm_set_eps_elem(lfasm_handle *h, const double *eps)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   if (!eps)
+   {
+      dfree(h->eps_elem);
+      return LFASM_OK;
+   }
+   LF_HIP(upload(h, h->eps_elem, (const double2 *)eps, (size_t)h->ne));
+   return LFASM_OK;
+}
+
+int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t nnz,
+                       const unsigned char *is_ess)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   if (!I || !Jcol || nnz <= 0 || nnz > INT32_MAX)
+   {
+      return h->fail(LFASM_E_ARG, "CSR pattern missing or nnz out of int32 range");
+   }
+   const int n = h->height;
+   if (I[0] != 0 || I[n] != nnz) { return h->fail(LFASM_E_ARG, "I[0] != 0 or I[n] != nnz"); }
+   int maxrow = 0;
+   for (int r = 0; r < n; ++r)
+   {
+      const int len = I[r + 1] - I[r];
+      if (len < 0) { return h->fail(LFASM_E_ARG, "I not monotone at row %d", r); }
+      maxrow = std::max(maxrow, len);
+   }
+   if (maxrow > 65535) { return h->fail(LFASM_E_UNSUPPORTED, "row length %d > 65535", maxrow); }
+   for (int64_t k = 0; k < nnz; ++k)
+   {
+      if (Jcol[k] < 0 || Jcol[k] >= n) { return h->fail(LFASM_E_ARG, "Jcol[%lld] out of range", (long long)k); }
+   }
+   LF_HIP(hipSetDevice(h->device));
+   release_global(h);
+   h->nnz = nnz;
+   h->maxrow = maxrow;
+   // per-row contributor lists (element, local index), ascending element
+   // order: counting sort of the element dof table
+   const size_t ne = (size_t)h->ne, nd = (size_t)h->nd;
+   std::vector<int> dof(ne * nd);
+   LF_HIP(hipMemcpy(dof.data(), h->dof, dof.size() * sizeof(int), hipMemcpyDeviceToHost));
+   std::vector<int> rc_ptr((size_t)n + 1, 0), rc_list(ne * nd);
+   for (size_t i = 0; i < ne * nd; ++i) { rc_ptr[(size_t)dof[i] + 1]++; }
+   for (int r = 0; r < n; ++r) { rc_ptr[(size_t)r + 1] += rc_ptr[r]; }
+   {
+      std::vector<int> pos(rc_ptr.begin(), rc_ptr.end() - 1);
+      for (size_t i = 0; i < ne * nd; ++i) { rc_list[(size_t)pos[dof[i]]++] = (int)i; }
+   }
+   LF_HIP(upload(h, h->Ip, I, (size_t)n + 1));
+   LF_HIP(upload(h, h->Jc, Jcol, (size_t)nnz));
+   LF_HIP(upload(h, h->rc_ptr, rc_ptr.data(), rc_ptr.size()));
+   LF_HIP(upload(h, h->rc_list, rc_list.data(), rc_list.size()));
+   std::vector<unsigned char> ess((size_t)n, 0);
+   if (is_ess) { std::memcpy(ess.data(), is_ess, (size_t)n); }
+   LF_HIP(upload(h, h->ess, ess.data(), ess.size()));
+   LF_HIP(hipMalloc((void **)&h->vals, (size_t)nnz * sizeof(double2)));
+   LF_HIP(hipMalloc((void **)&h->slot, ne * nd * nd * sizeof(uint16_t)));
+   LF_HIP(hipMalloc((void **)&h->bad, sizeof(int)));
+   LF_HIP(hipMemsetAsync(h->bad, 0, sizeof(int), h->stream));
+   LF_HIP(ev_begin(h, T_BUILD));
+   hipLaunchKernelGGL(k_slot_map, dim3(2048), dim3(BLOCK), 0, h->stream, h->ne, h->nd, h->dof,
+                      h->Ip, h->Jc, h->slot, h->bad);
+   LF_HIP(hipGetLastError());
+   LF_HIP(ev_end(h, T_BUILD));
+   int bad = 0;
+   LF_HIP(hipMemcpyAsync(&bad, h->bad, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   if (bad)
+   {
+      release_global(h);
+      return h->fail(LFASM_E_PATTERN, "an element (row, col) pair is missing from the CSR pattern");
+   }
+   h->global_built = true;
+   return LFASM_OK;
+}
+
+int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te,
+                               int kernel)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   const LocalParams P = local_params(h, mask_cc, mask_mass, apply_te, h->Ae);
+   LF_HIP(ev_begin(h, T_LOCAL));
+   if (!P.use_cc && !P.mflags)
+   {
+      const size_t n = (size_t)h->ne * h->nd * h->nd;
+      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, h->Ae, n);
+      LF_HIP(hipGetLastError());
+   }
+   else
+   {
+      LF_HIP(launch_local(h, P, kernel));
+   }
+   LF_HIP(ev_end(h, T_LOCAL));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   h->local_valid = true;
+   return LFASM_OK;
+}
+
+int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te)
+{
+   return lfasm_compute_local_kernel(h, mask_cc, mask_mass, apply_te, 0);
+}
+
+int lfasm_scatter_eliminate(lfasm_handle *h, int eliminate)
+{
+   if (!h || !h->global_built || !h->local_valid)
+   {
+      return h ? h->fail(LFASM_E_STATE, "build_global and compute_local first") : LFASM_E_STATE;
+   }
+   LF_HIP(hipSetDevice(h->device));
+   ScatterParams S;
+   S.n = h->height;
+   S.nd = h->nd;
+   S.Ip = h->Ip;
+   S.Jc = h->Jc;
+   S.dof = h->dof;
+   S.sign = h->sign;
+   S.rc_ptr = h->rc_ptr;
+   S.rc_list = h->rc_list;
+   S.slot = h->slot;
+   S.is_ess = h->ess;
+   S.Ae = h->Ae;
+   S.vals = h->vals;
+   S.eliminate = eliminate ? 1 : 0;
+   S.maxrow = h->maxrow;
+   const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
+   LF_HIP(hipFuncSetAttribute((const void *)k_scatter_rows,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+   LF_HIP(ev_begin(h, T_SCATTER));
+   hipLaunchKernelGGL(k_scatter_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem, h->stream, S);
+   LF_HIP(hipGetLastError());
+   LF_HIP(ev_end(h, T_SCATTER));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   return LFASM_OK;
+}
+
+int lfasm_device_csr(lfasm_handle *h, int **I, int **Jcol, void **vals)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   if (I) { *I = h->Ip; }
+   if (Jcol) { *Jcol = h->Jc; }
+   if (vals) { *vals = (void *)h->vals; }
+   return LFASM_OK;
+}
+
+int lfasm_download_values(lfasm_handle *h, double *re, double *im)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   std::vector<double2> v((size_t)h->nnz);
+   LF_HIP(hipMemcpyAsync(v.data(), h->vals, v.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                         h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   for (size_t k = 0; k < v.size(); ++k)
+   {
+      if (re) { re[k] = v[k].x; }
+      if (im) { im[k] = v[k].y; }
+   }
+   return LFASM_OK;
+}
+
+int lfasm_download_local(lfasm_handle *h, double *Ae)
+{
+   if (!h || !h->inited || !Ae) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   std::printf("asm_gpu_test_download (test-only D2H)\n");
+   std::fflush(stdout);
+   LF_HIP(hipSetDevice(h->device));
+   LF_HIP(hipMemcpyAsync(Ae, h->Ae, (size_t)h->ne * h->nd * h->nd * sizeof(double2),
+                         hipMemcpyDeviceToHost, h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   return LFASM_OK;
+}
+
+int lfasm_residual(lfasm_handle *h, const double *X2n, const double *B2n, double *rel)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   if (!X2n || !B2n || !rel) { return h->fail(LFASM_E_ARG, "null vector"); }
+   LF_HIP(hipSetDevice(h->device));
+   const int n = h->height;
+   std::vector<double2> hx((size_t)n), hy((size_t)n);
+   for (int i = 0; i < n; ++i) { hx[(size_t)i] = make_double2(X2n[i], X2n[n + i]); }
+   double2 *dx = nullptr, *dy = nullptr;
+   LF_HIP(hipMalloc((void **)&dx, (size_t)n * sizeof(double2)));
+   hipError_t e = hipMalloc((void **)&dy, (size_t)n * sizeof(double2));
+   if (e == hipSuccess)
+   {
+      e = hipMemcpyAsync(dx, hx.data(), (size_t)n * sizeof(double2), hipMemcpyHostToDevice, h->stream);
+   }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_spmv, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, h->Ip, h->Jc,
+                         h->vals, dx, dy);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess)
+   {
+      e = hipMemcpyAsync(hy.data(), dy, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, h->stream);
+   }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   (void)hipFree(dx);
+   if (dy) { (void)hipFree(dy); }
+   LF_HIP(e);
+   double rn = 0.0, bn = 0.0;
+   for (int i = 0; i < n; ++i)
+   {
+      const double dr = hy[(size_t)i].x - B2n[i], di = hy[(size_t)i].y - B2n[n + i];
+      rn += dr * dr + di * di;
+      bn += B2n[i] * B2n[i] + B2n[n + i] * B2n[n + i];
+   }
+   *rel = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+   return LFASM_OK;
+}
+
+int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (ne) { *ne = h->ne; }
+   if (nd) { *nd = h->nd; }
+   if (n) { *n = h->height; }
+   if (nnz) { *nnz = h->nnz; }
+   return LFASM_OK;
+}
+
+int lfasm_timings(const lfasm_handle *hc, double *ms, int n)
+{
+   lfasm_handle *h = const_cast<lfasm_handle *>(hc);
+   if (!h || !ms) { return LFASM_E_ARG; }
+   for (int t = 0; t < n && t < T_N; ++t)
+   {
+      ms[t] = 0.0;
+      if (!h->ev_used[t]) { continue; }
+      LF_HIP(hipEventSynchronize(h->ev[2 * t + 1]));
+      float f = 0.0f;
+      LF_HIP(hipEventElapsedTime(&f, h->ev[2 * t], h->ev[2 * t + 1]));
+      ms[t] = f;
+   }
+   return LFASM_OK;
+}
+
+void *lfasm_stream(lfasm_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+// build_csr_symbolic (solver/asm_data.inc:178-210) on host threads: rows are
+// split into contiguous ranges; each row's column list is the sorted union
+// of the dof lists of the elements that contain the row.
+int64_t lfasm_csr_symbolic(int ne, int nd, int height, const int *dof, int *I, int **Jcol)
+{
+   if (ne <= 0 || nd <= 0 || height <= 0 || !dof || !I || !Jcol) { return LFASM_E_ARG; }
+   const size_t nn = (size_t)ne * nd;
+   for (size_t i = 0; i < nn; ++i)
+   {
+      if (dof[i] < 0 || dof[i] >= height) { return LFASM_E_ARG; }
+   }
+   std::vector<int> cptr((size_t)height + 1, 0), elist(nn);
+   for (size_t i = 0; i < nn; ++i) { cptr[(size_t)dof[i] + 1]++; }
+   for (int r = 0; r < height; ++r) { cptr[(size_t)r + 1] += cptr[r]; }
+   {
+      std::vector<int> pos(cptr.begin(), cptr.end() - 1);
+      for (size_t i = 0; i < nn; ++i) { elist[(size_t)pos[dof[i]]++] = (int)(i / nd); }
+   }
+   unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+   std::vector<int> len((size_t)height, 0);
+   auto row_cols = [&](int r, std::vector<int> &tmp) {
+      tmp.clear();
+      for (int c = cptr[r]; c < cptr[(size_t)r + 1]; ++c)
+      {
+         const int *ed = dof + (size_t)elist[c] * nd;
+         tmp.insert(tmp.end(), ed, ed + nd);
+      }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+   };
+   auto run = [&](auto body) {
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nth; ++t)
+      {
+         th.emplace_back([&, t]() {
+            std::vector<int> tmp;
+            const int r0 = (int)((int64_t)height * t / nth), r1 = (int)((int64_t)height * (t + 1) / nth);
+            for (int r = r0; r < r1; ++r) { body(r, tmp); }
+         });
+      }
+      for (auto &x : th) { x.join(); }
+   };
+   run([&](int r, std::vector<int> &tmp) { row_cols(r, tmp); len[(size_t)r] = (int)tmp.size(); });
+   int64_t nnz = 0;
+   I[0] = 0;
+   for (int r = 0; r < height; ++r)
+   {
+      nnz += len[(size_t)r];
+      if (nnz > INT32_MAX) { return LFASM_E_UNSUPPORTED; }
+      I[r + 1] = (int)nnz;
+   }
+   int *J = (int *)std::malloc(sizeof(int) * (size_t)std::max<int64_t>(nnz, 1));
+   if (!J) { return LFASM_E_ARG; }
+   run([&](int r, std::vector<int> &tmp) {
+      row_cols(r, tmp);
+      std::copy(tmp.begin(), tmp.end(), J + I[r]);
+   });
+   *Jcol = J;
+   return nnz;
+}
+
+void lfasm_free(void *p) { std::free(p); }
+
+}  // extern "C"

@@ -178,10 +178,56 @@ __device__ __forceinline__ MassCoef mass_coef(const LocalParams &P, double2 eps)
    return c;
 }
 
+// Element constants kept in LDS during the K stream (EC_SIZE doubles):
+// [0..8] J/detJ, [9..17] J^{-1}, [18..28] MassCoef.
+constexpr int EC_SIZE = 32;
+
+__device__ __forceinline__ void store_elem_const(const ElemGeo &g, const MassCoef &mc,
+                                                 double *ec)
+{
+#pragma unroll
+   for (int i = 0; i < 9; ++i)
+   {
+      ec[i] = g.Jd[i];
+      ec[9 + i] = g.Ji[i];
+   }
+   ec[18] = mc.m0r; ec[19] = mc.m0i; ec[20] = mc.m1r; ec[21] = mc.m1i;
+   ec[22] = mc.m2r; ec[23] = mc.m2i; ec[24] = mc.kxy; ec[25] = mc.c0x;
+   ec[26] = mc.c0y; ec[27] = mc.c1x; ec[28] = mc.c1y;
+}
+
+// Element setup for one element group: per-point factors into qcoef (nqt x 4)
+// and the element constants into ec.  Threads tid < nthr of the group.
+__device__ __forceinline__ void elem_setup(const LocalParams &P, int e, int nqt, int tid,
+                                           int nthr, double *qcoef, double *ec)
+{
+   ElemGeo g;
+   load_geo(P, e, g);
+   for (int q = tid; q < nqt; q += nthr) { q_coef(P, g, q, qcoef + q * 4); }
+   if (tid == 0) { store_elem_const(g, mass_coef(P, g.eps), ec); }
+}
+
+// Contravariant Piola c = (J/detJ) chat, covariant v = J^{-T} vhat.
+__device__ __forceinline__ void piola_c(const volatile double *ec, double h0, double h1, double h2,
+                                        double &c0, double &c1, double &c2)
+{
+   c0 = ec[0] * h0 + ec[1] * h1 + ec[2] * h2;
+   c1 = ec[3] * h0 + ec[4] * h1 + ec[5] * h2;
+   c2 = ec[6] * h0 + ec[7] * h1 + ec[8] * h2;
+}
+
+__device__ __forceinline__ void piola_v(const volatile double *ec, double a0, double a1, double a2,
+                                        double &v0, double &v1, double &v2)
+{
+   v0 = a0 * ec[9] + a1 * ec[12] + a2 * ec[15];
+   v1 = a0 * ec[10] + a1 * ec[13] + a2 * ec[16];
+   v2 = a0 * ec[11] + a1 * ec[14] + a2 * ec[17];
+}
+
 // Form one (point, basis k) item of a chunk into the LDS staging rows
 // A[row][k], Cr[row][k], Ci[row][k] (row stride NP).  qc = per-point
 // factors (wa.re, wa.im, wb.re, wb.im).  valid == false writes zero rows.
-__device__ __forceinline__ void form_cc_item(const LocalParams &P, const ElemGeo &g,
+__device__ __forceinline__ void form_cc_item(const LocalParams &P, const double *ec,
                                              const double *qc, int q, int k, bool valid,
                                              double *A, double *Cr, double *Ci, int row0,
                                              int NP)
@@ -190,10 +236,7 @@ __device__ __forceinline__ void form_cc_item(const LocalParams &P, const ElemGeo
    if (valid && k < P.nd)
    {
       const double *cs = P.rc.cs + ((size_t)q * P.nd + k) * 3;
-      const double h0 = cs[0], h1 = cs[1], h2 = cs[2];
-      c0 = g.Jd[0] * h0 + g.Jd[1] * h1 + g.Jd[2] * h2;
-      c1 = g.Jd[3] * h0 + g.Jd[4] * h1 + g.Jd[5] * h2;
-      c2 = g.Jd[6] * h0 + g.Jd[7] * h1 + g.Jd[8] * h2;
+      piola_c(ec, cs[0], cs[1], cs[2], c0, c1, c2);
    }
    const double war = valid ? qc[0] : 0.0, wai = valid ? qc[1] : 0.0;
    const double wbr = valid ? qc[2] : 0.0, wbi = valid ? qc[3] : 0.0;
@@ -208,35 +251,28 @@ __device__ __forceinline__ void form_cc_item(const LocalParams &P, const ElemGeo
    Ci[(row0 + 2) * NP + k] = wbi * c2;
 }
 
-__device__ __forceinline__ void form_mass_item(const LocalParams &P, const ElemGeo &g,
-                                               const MassCoef &mc, const double *qc,
-                                               int q, int k, bool valid, int rows,
-                                               double *A, double *Cr, double *Ci,
+__device__ __forceinline__ void form_mass_item(const LocalParams &P, const double *ec,
+                                               const double *qc, int q, int k, bool valid,
+                                               int rows, double *A, double *Cr, double *Ci,
                                                int row0, int NP)
 {
    double v0 = 0.0, v1 = 0.0, v2 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
    if (valid && k < P.nd)
    {
       const double *vs = P.rm.vs + ((size_t)q * P.nd + k) * 3;
-      const double a0 = vs[0], a1 = vs[1], a2 = vs[2];
-      // covariant Piola v_i = sum_j vhat_j Jinv[j][i]
-      v0 = a0 * g.Ji[0] + a1 * g.Ji[3] + a2 * g.Ji[6];
-      v1 = a0 * g.Ji[1] + a1 * g.Ji[4] + a2 * g.Ji[7];
-      v2 = a0 * g.Ji[2] + a1 * g.Ji[5] + a2 * g.Ji[8];
+      piola_v(ec, vs[0], vs[1], vs[2], v0, v1, v2);
       if (rows == 6)
       {
          const double *cs = P.rm.cs + ((size_t)q * P.nd + k) * 3;
-         const double h0 = cs[0], h1 = cs[1], h2 = cs[2];
-         c0 = g.Jd[0] * h0 + g.Jd[1] * h1 + g.Jd[2] * h2;
-         c1 = g.Jd[3] * h0 + g.Jd[4] * h1 + g.Jd[5] * h2;
-         c2 = g.Jd[6] * h0 + g.Jd[7] * h1 + g.Jd[8] * h2;
+         piola_c(ec, cs[0], cs[1], cs[2], c0, c1, c2);
       }
    }
    const double2 wa = valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
    const double2 wb = valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
-   const double2 al0 = make_double2(mc.m0r * v0 - mc.kxy * v1, mc.m0i * v0 + mc.c0y * c2);
-   const double2 al1 = make_double2(mc.m1r * v1 - mc.kxy * v0, mc.m1i * v1 - mc.c0x * c2);
-   const double2 al2 = make_double2(mc.m2r * v2, mc.m2i * v2 - mc.c0y * c0 + mc.c0x * c1);
+   const volatile double *mc = ec + 18;   // m0r m0i m1r m1i m2r m2i kxy c0x c0y c1x c1y
+   const double2 al0 = make_double2(mc[0] * v0 - mc[6] * v1, mc[1] * v0 + mc[8] * c2);
+   const double2 al1 = make_double2(mc[2] * v1 - mc[6] * v0, mc[3] * v1 - mc[7] * c2);
+   const double2 al2 = make_double2(mc[4] * v2, mc[5] * v2 - mc[8] * c0 + mc[7] * c1);
    const double2 C0 = cmul(wb, al0), C1 = cmul(wb, al1), C2 = cmul(wa, al2);
    A[(row0 + 0) * NP + k] = v0;
    A[(row0 + 1) * NP + k] = v1;
@@ -249,9 +285,9 @@ __device__ __forceinline__ void form_mass_item(const LocalParams &P, const ElemG
    Ci[(row0 + 2) * NP + k] = C2.y;
    if (rows == 6)
    {
-      const double2 al3 = make_double2(0.0, mc.c1y * v2);
-      const double2 al4 = make_double2(0.0, -mc.c1x * v2);
-      const double2 al5 = make_double2(0.0, -mc.c1y * v0 + mc.c1x * v1);
+      const double2 al3 = make_double2(0.0, mc[10] * v2);
+      const double2 al4 = make_double2(0.0, -mc[9] * v2);
+      const double2 al5 = make_double2(0.0, -mc[10] * v0 + mc[9] * v1);
       const double2 C3 = cmul(wa, al3), C4 = cmul(wa, al4), C5 = cmul(wb, al5);
       A[(row0 + 3) * NP + k] = c0;
       A[(row0 + 4) * NP + k] = c1;
@@ -268,10 +304,10 @@ __device__ __forceinline__ void form_mass_item(const LocalParams &P, const ElemG
 // Form chunk `ch` of the combined K stream (cc chunks first, then mass
 // chunks) for the element group: items (point, basis) strided over nthr
 // threads starting at tid.  Staging: A, Cr, Ci of CHUNK_ROWS x NP each.
-__device__ __forceinline__ void form_chunk(const LocalParams &P, const ElemGeo &g,
-                                           const MassCoef &mc, const StreamShape &ss,
-                                           const double *qcoef, int ch, int tid, int nthr,
-                                           double *A, double *Cr, double *Ci, int NP)
+__device__ __forceinline__ void form_chunk(const LocalParams &P, const double *ec,
+                                           const StreamShape &ss, const double *qcoef,
+                                           int ch, int tid, int nthr, double *A, double *Cr,
+                                           double *Ci, int NP)
 {
    if (ch < ss.nch_cc)
    {
@@ -281,8 +317,8 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const ElemGeo &
          const int qi = it / NP, k = it - qi * NP;
          const int q = ch * ss.qpc_cc + qi;
          const bool valid = q < ss.nq_cc;
-         form_cc_item(P, g, qcoef + (size_t)(valid ? q : 0) * 4, valid ? q : 0, k, valid,
-                      A, Cr, Ci, qi * 3, NP);
+         const int qs = valid ? q : 0;
+         form_cc_item(P, ec, qcoef + (size_t)qs * 4, qs, k, valid, A, Cr, Ci, qi * 3, NP);
       }
    }
    else
@@ -295,47 +331,50 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const ElemGeo &
          const int q = c * ss.qpc_m + qi;
          const bool valid = q < ss.nq_m;
          const int qs = valid ? q : 0;
-         form_mass_item(P, g, mc, qcoef + (size_t)(P.rc.nq + qs) * 4, qs, k, valid,
-                        ss.rows_m, A, Cr, Ci, qi * ss.rows_m, NP);
+         form_mass_item(P, ec, qcoef + (size_t)(P.rc.nq + qs) * 4, qs, k, valid, ss.rows_m,
+                        A, Cr, Ci, qi * ss.rows_m, NP);
       }
    }
 }
 
 // ---- MFMA local-matrix kernel ---------------------------------------------
 // v_mfma_f64_16x16x4_f64: A 16x4, B 4x16, D 16x16.  Lane l holds
-// A[l%16][l/16], B[l/16][l%16] and D[4(l/16)+v][l%16], v = 0..3.
+// A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4v][l%16], v = 0..3 (the FP64
+// output interleaves rows across the four 16-lane groups).
 // Ae[k][l] = sum_K Bst[K][k] C[K][l]: A operand = staged shape rows,
 // B operand = staged coefficient rows (re and im separately).
 //
 // NT = column tiles (nd padded to 16 NT), RT = row tiles per wavefront,
-// WPE = NT / RT wavefronts per element, 4 / WPE elements per block.
+// WPE = NT / RT wavefronts per element, EPB elements per block; the block
+// has 64 WPE EPB threads.  WAVES_PER_EU bounds the register budget.
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-template <int NT, int RT>
-__global__ void __launch_bounds__(BLOCK) k_local_mfma(LocalParams P, int nqt)
+template <int NT, int RT, int EPB, int WAVES_PER_EU>
+__global__ void __launch_bounds__(64 * (NT / RT) * EPB)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+k_local_mfma(LocalParams P, int nqt)
 {
    constexpr int NP = 16 * NT;
    constexpr int WPE = NT / RT;
-   constexpr int EPB = 4 / WPE;
-   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   static_assert(NT % RT == 0, "row tiles must divide the tile count");
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
    const int slot = wave / WPE, wsub = wave - slot * WPE;
    const int e_raw = blockIdx.x * EPB + slot;
    const bool live = e_raw < P.ne;
    const int e = live ? e_raw : P.ne - 1;
 
    extern __shared__ double smem[];
-   const int slot_doubles = nqt * 4 + 3 * CHUNK_ROWS * NP;
-   double *qcoef = smem + (size_t)slot * slot_doubles;
+   const int slot_doubles = EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP;
+   double *ec = smem + (size_t)slot * slot_doubles;
+   double *qcoef = ec + EC_SIZE;
    double *A = qcoef + nqt * 4;
    double *Cr = A + CHUNK_ROWS * NP;
    double *Ci = Cr + CHUNK_ROWS * NP;
    const int gtid = wsub * 64 + lane;   // thread index within the element group
 
-   ElemGeo g;
-   load_geo(P, e, g);
-   for (int q = gtid; q < nqt; q += WPE * 64) { q_coef(P, g, q, qcoef + q * 4); }
-   const MassCoef mc = mass_coef(P, g.eps);
+   elem_setup(P, e, nqt, gtid, WPE * 64, qcoef, ec);
    const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
    __syncthreads();
 
@@ -353,9 +392,9 @@ __global__ void __launch_bounds__(BLOCK) k_local_mfma(LocalParams P, int nqt)
    const int nch = ss.nch_cc + ss.nch_m;
    for (int ch = 0; ch < nch; ++ch)
    {
-      form_chunk(P, g, mc, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
+      form_chunk(P, ec, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
       __syncthreads();
-#pragma unroll
+#pragma unroll 1
       for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
       {
          const int row = ks * 4 + g4;
@@ -411,28 +450,32 @@ __global__ void __launch_bounds__(BLOCK) k_local_mfma(LocalParams P, int nqt)
                   acc[t][j][c][v] = cs * x + cp * y;
                }
       }
+      // rows: row = 16 I + g4 + 4 v, so the pair (d0, d0+1) of a face
+      // sits in lanes l (g4 even) and l ^ 16 (g4 odd), same register
 #pragma unroll
       for (int t = 0; t < RT; ++t)
-      {
 #pragma unroll
-         for (int vp = 0; vp < 2; ++vp)
+         for (int v = 0; v < 4; ++v)
          {
-            const int row = 16 * (wsub * RT + t) + 4 * g4 + 2 * vp;   // even row
+            const int row = 16 * (wsub * RT + t) + g4 + 4 * v;
+            double cs = 1.0, cp = 0.0;
             if (row >= foff && row < foff + 4 * ntd)
             {
                const double *Ti = P.TInv + 4 * fo[(row - foff) / ntd];
-#pragma unroll
-               for (int j = 0; j < NT; ++j)
-#pragma unroll
-                  for (int c = 0; c < 2; ++c)
-                  {
-                     const double a0 = acc[t][j][c][2 * vp], b0 = acc[t][j][c][2 * vp + 1];
-                     acc[t][j][c][2 * vp] = Ti[0] * a0 + Ti[1] * b0;
-                     acc[t][j][c][2 * vp + 1] = Ti[2] * a0 + Ti[3] * b0;
-                  }
+               const bool even = ((row - foff) & 1) == 0;
+               cs = even ? Ti[0] : Ti[3];
+               cp = even ? Ti[1] : Ti[2];
             }
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+               for (int c = 0; c < 2; ++c)
+               {
+                  const double x = acc[t][j][c][v];
+                  const double y = __shfl_xor(x, 16);
+                  acc[t][j][c][v] = cs * x + cp * y;
+               }
          }
-      }
    }
 
    if (!live) { return; }
@@ -446,7 +489,7 @@ __global__ void __launch_bounds__(BLOCK) k_local_mfma(LocalParams P, int nqt)
 #pragma unroll
          for (int v = 0; v < 4; ++v)
          {
-            const int row = 16 * (wsub * RT + t) + 4 * g4 + v;
+            const int row = 16 * (wsub * RT + t) + g4 + 4 * v;
             if (row < nd && col < nd)
             {
                Ae[(size_t)row * nd + col] = make_double2(acc[t][j][0][v], acc[t][j][1][v]);
@@ -460,21 +503,20 @@ __global__ void __launch_bounds__(BLOCK) k_local_mfma(LocalParams P, int nqt)
 // the entries idx = tid + BLOCK*i of the nd x nd matrix.  The face-DOF
 // transform runs afterwards in k_te.
 
-template <int NACC>
-__global__ void __launch_bounds__(BLOCK) k_local_generic(LocalParams P, int nqt, int NP)
+template <int NACC, int NTHR>
+__global__ void __launch_bounds__(NTHR)
+k_local_generic(LocalParams P, int nqt, int NP)
 {
    const int e = blockIdx.x;
    const int tid = threadIdx.x;
    extern __shared__ double smem[];
-   double *qcoef = smem;
+   double *ec = smem;
+   double *qcoef = ec + EC_SIZE;
    double *A = qcoef + nqt * 4;
    double *Cr = A + CHUNK_ROWS * NP;
    double *Ci = Cr + CHUNK_ROWS * NP;
 
-   ElemGeo g;
-   load_geo(P, e, g);
-   for (int q = tid; q < nqt; q += BLOCK) { q_coef(P, g, q, qcoef + q * 4); }
-   const MassCoef mc = mass_coef(P, g.eps);
+   elem_setup(P, e, nqt, tid, NTHR, qcoef, ec);
    const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
    __syncthreads();
 
@@ -486,12 +528,12 @@ __global__ void __launch_bounds__(BLOCK) k_local_generic(LocalParams P, int nqt,
    const int nch = ss.nch_cc + ss.nch_m;
    for (int ch = 0; ch < nch; ++ch)
    {
-      form_chunk(P, g, mc, ss, qcoef, ch, tid, BLOCK, A, Cr, Ci, NP);
+      form_chunk(P, ec, ss, qcoef, ch, tid, NTHR, A, Cr, Ci, NP);
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < NACC; ++i)
       {
-         const int idx = tid + BLOCK * i;
+         const int idx = tid + NTHR * i;
          if (idx < n2)
          {
             const int k = idx / nd, l = idx - k * nd;
@@ -512,7 +554,7 @@ __global__ void __launch_bounds__(BLOCK) k_local_generic(LocalParams P, int nqt,
 #pragma unroll
    for (int i = 0; i < NACC; ++i)
    {
-      const int idx = tid + BLOCK * i;
+      const int idx = tid + NTHR * i;
       if (idx < n2) { Ae[idx] = make_double2(ar[i], ai[i]); }
    }
 }

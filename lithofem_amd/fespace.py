@@ -202,3 +202,36 @@ def csr_symbolic(dof: np.ndarray, height: int) -> tuple[np.ndarray, np.ndarray]:
     I = np.zeros(height + 1, dtype=np.int32)
     np.add.at(I, rows + 1, 1)
     return np.cumsum(I).astype(np.int32), cols
+
+
+def interpolate(ad: AsmData, field) -> np.ndarray:
+    """Global DOF vector of a vector field F(x) (callable: (n,3) points ->
+    (n,3) values, real or complex): every element evaluates its local
+    functionals (tangential components at its DOF points, physical edge and
+    face tangents), then maps them to the global numbering with the inverse
+    of the element transform (sign, face-pair TInv block).  For a field in
+    the discrete space the result is exact; fields with a tangential jump
+    across elements get the value of the last element written."""
+    from .nedelec import ND_TINV, dof_functionals
+    pts, tans = dof_functionals(ad.p)
+    ne, nd, p = ad.ne, ad.nd, ad.p
+    J = ad.J.reshape(ne, 3, 3)
+    X = ad.v0.reshape(ne, 1, 3) + np.einsum("eij,kj->eki", J, pts)
+    T = np.einsum("eij,kj->eki", J, tans)
+    F = np.asarray(field(X.reshape(-1, 3))).reshape(ne, nd, 3)
+    loc = np.einsum("ekd,ekd->ek", F, T)
+    if p >= 2:
+        ntd = p * (p - 1)
+        Ti = ND_TINV.reshape(6, 4)
+        # l = M g with M = [[t0, t2], [t1, t3]]  ->  g = M^{-1} l
+        M = np.stack([np.stack([Ti[:, 0], Ti[:, 2]], -1), np.stack([Ti[:, 1], Ti[:, 3]], -1)], 1)
+        Minv = np.linalg.inv(M)                                  # (6, 2, 2)
+        for f in range(4):
+            Mi = Minv[ad.fo[:, f]]                               # (ne, 2, 2)
+            for i in range(ntd // 2):
+                d0 = 6 * p + f * ntd + 2 * i
+                pair = loc[:, d0:d0 + 2].copy()
+                loc[:, d0:d0 + 2] = np.einsum("eij,ej->ei", Mi, pair)
+    g = np.zeros(ad.height, dtype=loc.dtype)
+    g[ad.dof.ravel()] = (loc * ad.sign).ravel()
+    return g

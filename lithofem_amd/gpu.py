@@ -1,0 +1,256 @@
+"""Python mirror of the reference's AsmGpu / AsmGpuGlobal interface over the
+C-ABI of include/lithofem_asm.h (liblfasm.so, built by lithofem_amd.build).
+
+Method names follow solver/asm_gpu.inc:271-599 (Init, ComputeLocal,
+BuildGlobal, ScatterEliminate, Residual, DeviceI/J/Values, DownloadLocal,
+DownloadValues) in snake_case.  Errors raise :class:`AsmGpuError` carrying
+the library's status code and message -- the caller falls back to CPU
+assembly exactly where the reference driver does
+(solver/lithofem_solve.cpp:796-799).  There is no CPU fallback inside this
+module: without the built library every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+from .nedelec import ND_TINV
+
+LIB_PATH = Path(__file__).resolve().parent / "liblfasm.so"
+
+CURLCURL, MASS, CROSS0, CROSS1, KLK = 1, 2, 4, 8, 16
+MASS_ALL = MASS | CROSS0 | CROSS1 | KLK
+
+E_ARG, E_HIP, E_FORCED, E_UNSUPPORTED, E_STATE, E_PATTERN = -1, -2, -3, -4, -5, -6
+
+# kernel selector of lfasm_compute_local_kernel
+KERNEL_AUTO, KERNEL_VALU = 0, 1
+N_MFMA_VARIANTS = 4
+
+
+def kernel_mfma(variant: int = 0) -> int:
+    return 2 + variant
+
+
+class AsmGpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lfasm error {code}: {msg}")
+        self.code = code
+
+
+class _Rule(ct.Structure):
+    _fields_ = [("nq", ct.c_int), ("ip", ct.c_void_p), ("w", ct.c_void_p),
+                ("vshape", ct.c_void_p), ("curl", ct.c_void_p)]
+
+
+class _Pml(ct.Structure):
+    _fields_ = [(n, ct.c_double) for n in
+                ("z_min", "z_max", "thick", "order", "smax_bot", "smax_top", "k0", "kx", "ky")]
+
+
+class _Mesh(ct.Structure):
+    _fields_ = [("ne", ct.c_int), ("nd", ct.c_int), ("p", ct.c_int), ("height", ct.c_int),
+                ("dof", ct.c_void_p), ("sign", ct.c_void_p), ("fo", ct.c_void_p),
+                ("attr", ct.c_void_p), ("J", ct.c_void_p), ("Jinv", ct.c_void_p),
+                ("detJ", ct.c_void_p), ("v0", ct.c_void_p), ("rule_cc", _Rule),
+                ("rule_mass", _Rule), ("nreg", ct.c_int), ("eps_tab", ct.c_void_p),
+                ("eps_elem", ct.c_void_p), ("pml", _Pml), ("TInv", ct.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblfasm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise AsmGpuError(E_STATE, f"{LIB_PATH} not built (python -m lithofem_amd.build)")
+    # One HIP runtime per process: when PyTorch-ROCm is installed its
+    # bundled runtime is loaded first, and liblfasm.so binds to it (same
+    # soname) instead of bringing a second copy next to it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ct.CDLL(str(LIB_PATH))
+    vp, i, i64, d = ct.c_void_p, ct.c_int, ct.c_int64, ct.c_double
+    sig = {
+        "lfasm_create": ([i, ct.POINTER(vp)], i),
+        "lfasm_destroy": ([vp], None),
+        "lfasm_last_error": ([vp], ct.c_char_p),
+        "lfasm_pml_smax": ([d, d, d, d, d], d),
+        "lfasm_init": ([vp, vp], i),
+        "lfasm_set_eps_elem": ([vp, vp], i),
+        "lfasm_build_global": ([vp, vp, vp, i64, vp], i),
+        "lfasm_compute_local": ([vp, i, i, i], i),
+        "lfasm_compute_local_kernel": ([vp, i, i, i, i], i),
+        "lfasm_scatter_eliminate": ([vp, i], i),
+        "lfasm_device_csr": ([vp, vp, vp, vp], i),
+        "lfasm_download_values": ([vp, vp, vp], i),
+        "lfasm_download_local": ([vp, vp], i),
+        "lfasm_residual": ([vp, vp, vp, vp], i),
+        "lfasm_sizes": ([vp, vp, vp, vp, vp], i),
+        "lfasm_timings": ([vp, vp, i], i),
+        "lfasm_stream": ([vp], vp),
+        "lfasm_csr_symbolic": ([i, i, i, vp, vp, ct.POINTER(vp)], i64),
+        "lfasm_free": ([vp], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def csr_symbolic(dof: np.ndarray, height: int):
+    """Host multi-threaded CSR pattern build (lfasm_csr_symbolic)."""
+    ne, nd = dof.shape
+    dof = np.ascontiguousarray(dof, dtype=np.int32)
+    I = np.zeros(height + 1, dtype=np.int32)
+    pj = ct.c_void_p()
+    nnz = lib().lfasm_csr_symbolic(ne, nd, height, _p(dof), _p(I), ct.byref(pj))
+    if nnz < 0:
+        raise AsmGpuError(int(nnz), "csr_symbolic failed")
+    J = np.ctypeslib.as_array(ct.cast(pj, ct.POINTER(ct.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
+    lib().lfasm_free(pj)
+    return I, J
+
+
+class AsmGpuGlobal:
+    """One device assembly context (AsmGpu + AsmGpuGlobal of the reference)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ct.c_void_p()
+        rc = lib().lfasm_create(device, ct.byref(self._h))
+        if rc != 0:
+            msg = self.last_error()
+            self.close()
+            raise AsmGpuError(rc, msg)
+        self._keep = []
+        self.ne = self.nd = self.n = 0
+        self.nnz = 0
+
+    # -- lifetime
+    def close(self):
+        if self._h:
+            lib().lfasm_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def last_error(self) -> str:
+        s = lib().lfasm_last_error(self._h)
+        return s.decode() if s else ""
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise AsmGpuError(rc, self.last_error())
+
+    # -- AsmGpu::Init
+    def init(self, ad, problem, eps_elem=None):
+        keep = []
+
+        def k(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        def rule(r):
+            return _Rule(r.nq, _p(k(r.ip, np.float64)), _p(k(r.w, np.float64)),
+                         _p(k(r.vshape, np.float64)), _p(k(r.curl, np.float64)))
+
+        pml = problem.pml_params()
+        ee = None
+        if eps_elem is not None:
+            ee = k(np.asarray(eps_elem, dtype=np.complex128).view(np.float64), np.float64)
+        m = _Mesh(ad.ne, ad.nd, ad.p, ad.height,
+                  _p(k(ad.dof, np.int32)), _p(k(ad.sign, np.float64)),
+                  _p(k(ad.fo, np.int32)) if ad.p >= 2 else None,
+                  _p(k(ad.attr, np.int32)), _p(k(ad.J, np.float64)), _p(k(ad.Jinv, np.float64)),
+                  _p(k(ad.detJ, np.float64)), _p(k(ad.v0, np.float64)),
+                  rule(ad.rule_cc), rule(ad.rule_mass), len(problem.regions),
+                  _p(k(problem.eps_tab(), np.float64)), _p(ee), _Pml(*pml),
+                  _p(k(ND_TINV, np.float64)))
+        self._check(lib().lfasm_init(self._h, ct.byref(m)))
+        self.ne, self.nd, self.n = ad.ne, ad.nd, ad.height
+
+    def set_eps_elem(self, eps_elem):
+        if eps_elem is None:
+            self._check(lib().lfasm_set_eps_elem(self._h, None))
+            return
+        a = np.ascontiguousarray(np.asarray(eps_elem, dtype=np.complex128).view(np.float64))
+        self._check(lib().lfasm_set_eps_elem(self._h, _p(a)))
+
+    # -- AsmGpuGlobal::BuildGlobal
+    def build_global(self, I, Jcol, ess_list=None):
+        I = np.ascontiguousarray(I, dtype=np.int32)
+        Jcol = np.ascontiguousarray(Jcol, dtype=np.int32)
+        ess = np.zeros(self.n, dtype=np.uint8)
+        if ess_list is not None and len(ess_list):
+            ess[np.asarray(ess_list)] = 1
+        self._check(lib().lfasm_build_global(self._h, _p(I), _p(Jcol), len(Jcol), _p(ess)))
+        self.nnz = len(Jcol)
+
+    # -- AsmGpu::ComputeLocal
+    def compute_local(self, mask_cc=CURLCURL, mask_mass=MASS_ALL, apply_te=True,
+                      kernel=KERNEL_AUTO):
+        self._check(lib().lfasm_compute_local_kernel(self._h, mask_cc, mask_mass,
+                                                     1 if apply_te else 0, kernel))
+
+    # -- AsmGpuGlobal::ScatterEliminate
+    def scatter_eliminate(self, eliminate=True):
+        self._check(lib().lfasm_scatter_eliminate(self._h, 1 if eliminate else 0))
+
+    def download_local(self) -> np.ndarray:
+        out = np.empty((self.ne, self.nd, self.nd), dtype=np.complex128)
+        self._check(lib().lfasm_download_local(self._h, _p(out)))
+        return out
+
+    def download_values(self):
+        re = np.empty(self.nnz)
+        im = np.empty(self.nnz)
+        self._check(lib().lfasm_download_values(self._h, _p(re), _p(im)))
+        return re, im
+
+    def device_csr(self):
+        pi, pj, pv = ct.c_void_p(), ct.c_void_p(), ct.c_void_p()
+        self._check(lib().lfasm_device_csr(self._h, ct.byref(pi), ct.byref(pj), ct.byref(pv)))
+        return pi.value, pj.value, pv.value
+
+    def residual(self, X2n, B2n) -> float:
+        X2n = np.ascontiguousarray(X2n, dtype=np.float64)
+        B2n = np.ascontiguousarray(B2n, dtype=np.float64)
+        rel = ct.c_double()
+        self._check(lib().lfasm_residual(self._h, _p(X2n), _p(B2n), ct.byref(rel)))
+        return rel.value
+
+    def timings(self) -> dict:
+        ms = np.zeros(5)
+        self._check(lib().lfasm_timings(self._h, _p(ms), 5))
+        return dict(local_ms=ms[0], scatter_ms=ms[1], build_ms=ms[2], batch_ms=ms[3],
+                    adjoint_ms=ms[4])
+
+    def stream(self) -> int:
+        return lib().lfasm_stream(self._h) or 0

@@ -1,0 +1,171 @@
+"""GPU parity of the HIP assembly against the CPU oracle, through the C-ABI.
+
+Mirrors the reference's assembly harness (solver/asm_test.cpp, wrapped by
+tests/test_asm_v25.py):
+  U3  per-integrator element matrices (no face transform), p = 1..4, k_par on/off
+  U4  full element matrix + dual face-DOF transform, every element
+  U5  global CSR values (sorted-column pattern), no elimination
+  U6  eliminated system (essential rows/cols, unit diagonal)
+plus what the reference's atomics could not give: bitwise run-to-run
+determinism, and a scatter that is bit-identical to the oracle's
+element-order accumulation given the same element matrices.
+
+Criterion: max-normalised complex error < 1e-13 (tests/test_asm_v25.py:
+130-133,171-175).  Every kernel variant (VALU and all MFMA launch variants)
+is checked.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cases import case, maxnorm_err
+from lithofem_amd import gpu
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+KERNELS = [gpu.KERNEL_AUTO, gpu.KERNEL_VALU] + [gpu.kernel_mfma(v) for v in range(gpu.N_MFMA_VARIANTS)]
+
+
+def _sample(ad, n_per_attr=6):
+    out = []
+    for a in np.unique(ad.attr):
+        idx = np.nonzero(ad.attr == a)[0]
+        out.extend(idx[np.linspace(0, len(idx) - 1, min(n_per_attr, len(idx))).astype(int)])
+    return sorted(set(int(e) for e in out))
+
+
+def _ctx(c, ad, I=None, J=None, eps_elem=None):
+    g = gpu.AsmGpuGlobal(0)
+    g.init(ad, c.problem, eps_elem=eps_elem)
+    if I is not None:
+        g.build_global(I, J, ad.ess)
+    return g
+
+
+U3 = [("curlcurl", gpu.CURLCURL, 0, O.CURLCURL, False),
+      ("mass", 0, gpu.MASS, O.MASS, False),
+      ("cross0", 0, gpu.CROSS0, O.CROSS0, True),
+      ("cross1", 0, gpu.CROSS1, O.CROSS1, True),
+      ("klk", 0, gpu.KLK, O.KLK, True)]
+
+
+@pytest.mark.parametrize("theta", [0.0, 6.0], ids=["t0", "t6"])
+@pytest.mark.parametrize("p", [1, 2, 3, 4], ids=["p1", "p2", "p3", "p4"])
+@pytest.mark.parametrize("name", ["grating", "hole"])
+def test_u3_u4_local(name, p, theta):
+    size = "tiny"
+    c, ad, I, J = case(name, p, theta, size)
+    print(f"case {name} p={p} theta={theta} ne={ad.ne}", flush=True)
+    orc = O.Oracle(ad, c.problem, I, J)
+    sample = _sample(ad, 3 if p == 4 else 6)
+    kpar = c.problem.has_kpar
+    refs = {nm: np.stack([orc.element(e, om, False) for e in sample])
+            for nm, _, _, om, need_kpar in U3 if kpar or not need_kpar}
+    full = O.ALL if kpar else (O.CURLCURL | O.MASS)
+    ref_full = np.stack([orc.element(e, full, True) for e in sample])
+    g = _ctx(c, ad)
+    try:
+        for kern in KERNELS:
+            # U3: each integrator alone, no face transform
+            for nm, mcc, mm, om, need_kpar in U3:
+                if need_kpar and not kpar:
+                    continue
+                g.compute_local(mcc, mm, False, kernel=kern)
+                Ag = g.download_local()
+                err = maxnorm_err(Ag[sample], refs[nm])
+                assert err < TOL, (kern, nm, err)
+            # U4: full sum + face transform
+            g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=kern)
+            Ag = g.download_local()
+            err = maxnorm_err(Ag[sample], ref_full)
+            print(f"  kernel {kern}: u4 {err:.2e}", flush=True)
+            assert err < TOL, (kern, "u4", err)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("theta", [0.0, 6.0], ids=["t0", "t6"])
+@pytest.mark.parametrize("p", [1, 2, 3], ids=["p1", "p2", "p3"])
+@pytest.mark.parametrize("name", ["grating", "hole"])
+def test_u5_u6_global(name, p, theta):
+    c, ad, I, J = case(name, p, theta, "small")
+    print(f"case {name} p={p} theta={theta} ne={ad.ne} n={ad.height} nnz={len(J)}", flush=True)
+    orc = O.Oracle(ad, c.problem, I, J)
+    g = _ctx(c, ad, I, J)
+    try:
+        g.compute_local()
+        Ae = g.download_local()
+        # U5: scatter without elimination
+        g.scatter_eliminate(False)
+        vre, vim = g.download_values()
+        # bit-identical to the oracle's element-order scatter of the same Ae
+        sre, sim = orc.scatter(Ae)
+        assert np.array_equal(vre, sre) and np.array_equal(vim, sim)
+        # against the full CPU reassembly
+        rre, rim = orc.assemble()
+        err5 = maxnorm_err(vre + 1j * vim, rre + 1j * rim)
+        assert err5 < TOL, err5
+        # U6: essential elimination
+        g.scatter_eliminate(True)
+        ere, eim = g.download_values()
+        is_ess = np.zeros(ad.height, np.uint8)
+        is_ess[ad.ess] = 1
+        xre, xim = O.eliminate(I, J, is_ess, rre, rim)
+        err6 = maxnorm_err(ere + 1j * eim, xre + 1j * xim)
+        assert err6 < TOL, err6
+        # bitwise equal to eliminating the un-eliminated GPU values
+        yre, yim = O.eliminate(I, J, is_ess, vre, vim)
+        assert np.array_equal(ere, yre) and np.array_equal(eim, yim)
+        print(f"  u5 {err5:.2e} u6 {err6:.2e}", flush=True)
+        # determinism: a second full run is bitwise identical
+        g.compute_local()
+        g.scatter_eliminate(True)
+        e2r, e2i = g.download_values()
+        assert np.array_equal(ere, e2r) and np.array_equal(eim, e2i)
+        # residual against the oracle's formula
+        rng = np.random.default_rng(7)
+        X = rng.standard_normal(2 * ad.height)
+        B = rng.standard_normal(2 * ad.height)
+        rg = g.residual(X, B)
+        ro = O.residual(I, J, ere, eim, X, B)
+        assert abs(rg - ro) <= 1e-12 * ro
+    finally:
+        g.close()
+
+
+def test_force_fail_env(monkeypatch):
+    c, ad, I, J = case("grating", 2, 6.0, "tiny")
+    monkeypatch.setenv("LITHOFEM_ASM_FORCE_FAIL", "1")
+    g = gpu.AsmGpuGlobal(0)
+    try:
+        with pytest.raises(gpu.AsmGpuError) as ei:
+            g.init(ad, c.problem)
+        assert ei.value.code == gpu.E_FORCED
+        assert "forced failure injected" in str(ei.value)
+    finally:
+        g.close()
+
+
+def test_state_errors():
+    c, ad, I, J = case("grating", 1, 6.0, "tiny")
+    g = gpu.AsmGpuGlobal(0)
+    try:
+        with pytest.raises(gpu.AsmGpuError) as ei:
+            g.compute_local()
+        assert ei.value.code == gpu.E_STATE
+        g.init(ad, c.problem)
+        with pytest.raises(gpu.AsmGpuError) as ei:
+            g.scatter_eliminate(True)
+        assert ei.value.code == gpu.E_STATE
+        # a pattern that misses element pairs is rejected before any scatter
+        I2 = np.arange(ad.height + 1, dtype=np.int32)
+        J2 = np.arange(ad.height, dtype=np.int32)
+        with pytest.raises(gpu.AsmGpuError) as ei:
+            g.build_global(I2, J2, ad.ess)
+        assert ei.value.code == gpu.E_PATTERN
+    finally:
+        g.close()
