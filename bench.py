@@ -95,6 +95,12 @@ def main() -> int:
     g.build_global(I, J, ad.ess)
     t_upload = time.perf_counter() - t0
     log(f"upload+build_global {t_upload:.2f}s (device part {g.timings()['build_ms']:.2f} ms)")
+    tuned = None
+    if args.kernel == 0:
+        from lithofem_amd.tune import tune_local
+        tuned = tune_local(g)
+        log(f"autotune: best kernel {tuned['best']}: " +
+            ", ".join(f"{k}={v:.3f}ms" for k, v in tuned["ms"].items()))
 
     def step():
         g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=args.kernel)
@@ -143,6 +149,8 @@ def main() -> int:
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": None},
         "detail": {"local_kernel_ms": tl_avg, "scatter_ms": ts_avg,
+                   "local_kernel": tuned["best"] if tuned else args.kernel,
+                   "autotune_ms": {str(k): v for k, v in tuned["ms"].items()} if tuned else None,
                    "flops_per_element": W,
                    "nnz_per_s": len(J) / (ms_step * 1e-3),
                    "scatter_GBps": q_sc / (ts_avg * 1e-3) / 1e9,

@@ -129,6 +129,14 @@ int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_t
 int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te,
                                int kernel);
 
+/* Number of MFMA launch variants (kernel selectors 2 .. 2+N-1). */
+#define LFASM_N_MFMA_VARIANTS 4
+
+/* Make `kernel` (a selector of lfasm_compute_local_kernel) the one that
+ * selector 0 uses on this handle -- the launch autotuner's result.  0
+ * restores the built-in default.  No reference counterpart. */
+int lfasm_set_local_kernel(lfasm_handle *h, int kernel);
+
 /* AsmGpuGlobal::ScatterEliminate (solver/asm_gpu.inc:517-530): CSR values
  * from the local matrices by a deterministic per-row segmented reduction
  * (contributions summed in ascending element order), then (eliminate)
@@ -170,6 +178,13 @@ void *lfasm_stream(lfasm_handle *h);
  * with lfasm_free.  Returns nnz or a negative status. */
 int64_t lfasm_csr_symbolic(int ne, int nd, int height, const int *dof, int *I, int **Jcol);
 void lfasm_free(void *p);
+
+/* Measured device ceilings (no reference counterpart): out[0] FP64 MFMA
+ * TFLOP/s (v_mfma_f64_16x16x4f64 loop), out[1] FP64 VALU FMA TFLOP/s,
+ * out[2] HBM copy GB/s (2 GiB read + 2 GiB write), out[3] reported clock
+ * GHz, out[4] compute units.  Allocates 4 GiB of device memory while it
+ * runs. */
+int lfasm_measure_peaks(int device, double *out);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 // one HIP stream on its device.  Errors are returned as status codes with a
 // message string instead of printf + bool (solver/asm_gpu.inc:255-263).
 #include "lfasm_device.hpp"
+#include "lfasm_peaks.hpp"
 #include "../../include/lithofem_asm.h"
 
 #include <hip/hip_runtime.h>
@@ -52,12 +53,12 @@ struct lfasm_handle {
    int maxrow = 0;
    int *Ip = nullptr, *Jc = nullptr, *rc_ptr = nullptr, *rc_list = nullptr, *bad = nullptr;
    uint16_t *slot = nullptr;
-   unsigned char *ess = nullptr;
+   unsigned char *ess = nullptr, *eflag = nullptr;
    double2 *vals = nullptr;
    // timing
    hipEvent_t ev[2 * T_N] = {};
    bool ev_used[T_N] = {};
-   int mfma_variant = 0;
+   int local_kernel = 0;   // tuned choice used by kernel selector 0
 
    int fail(int code, const char *fmt, ...)
    {
@@ -117,7 +118,7 @@ void release_mesh(lfasm_handle *h)
 void release_global(lfasm_handle *h)
 {
    dfree(h->Ip); dfree(h->Jc); dfree(h->rc_ptr); dfree(h->rc_list);
-   dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->vals);
+   dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    h->global_built = false;
 }
 
@@ -212,25 +213,48 @@ hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
    return hipGetLastError();
 }
 
-// MFMA launch variants per tile count: (row tiles per wavefront, elements
-// per block, waves per EU).  Variant 0 of each list is the default; the
+template <int NT, int EPB, int WPEU>
+hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
+{
+   constexpr int NP = 16 * NT;
+   constexpr int THREADS = 64 * NT * EPB;
+   const int nqt = P.rc.nq + P.rm.nq;
+   const size_t shmem =
+      (size_t)EPB * (EC_SIZE + nqt * 4 + 2 * 3 * CHUNK_ROWS * NP) * sizeof(double);
+   hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_pipe<NT, EPB, WPEU>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   if (e != hipSuccess) { return e; }
+   const int grid = (P.ne + EPB - 1) / EPB;
+   hipLaunchKernelGGL((k_local_mfma_pipe<NT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
+                      h->stream, P, nqt);
+   return hipGetLastError();
+}
+
+// MFMA launch variants per tile count (element-per-block count, waves per
+// EU, pipelined or not).  Variant 0 of each list is the default; the
 // launch autotuner (lithofem_amd.tune) times all of them.
-constexpr int N_MFMA_VARIANTS = 4;
+constexpr int N_MFMA_VARIANTS = LFASM_N_MFMA_VARIANTS;
 
 hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, int v)
 {
    switch (nt * 16 + v)
    {
-   case 1 * 16 + 0: case 1 * 16 + 1: case 1 * 16 + 2: case 1 * 16 + 3:
-      return launch_mfma<1, 1, 4, 2>(h, P);
-   case 2 * 16 + 0: case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P);
-   case 2 * 16 + 1: case 2 * 16 + 3: return launch_mfma<2, 2, 4, 2>(h, P);
-   case 3 * 16 + 0: return launch_mfma<3, 1, 1, 2>(h, P);
-   case 3 * 16 + 1: return launch_mfma<3, 3, 4, 1>(h, P);
-   case 3 * 16 + 2: return launch_mfma<3, 1, 1, 3>(h, P);
-   case 3 * 16 + 3: return launch_mfma<3, 3, 2, 1>(h, P);
-   case 6 * 16 + 0: case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
-   case 6 * 16 + 1: case 6 * 16 + 3: return launch_mfma<6, 2, 1, 1>(h, P);
+   case 1 * 16 + 0: return launch_mfma_pipe<1, 4, 2>(h, P);
+   case 1 * 16 + 1: return launch_mfma_pipe<1, 2, 2>(h, P);
+   case 1 * 16 + 2: return launch_mfma<1, 1, 4, 2>(h, P);
+   case 1 * 16 + 3: return launch_mfma_pipe<1, 4, 4>(h, P);
+   case 2 * 16 + 0: return launch_mfma_pipe<2, 2, 2>(h, P);
+   case 2 * 16 + 1: return launch_mfma_pipe<2, 1, 2>(h, P);
+   case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P);
+   case 2 * 16 + 3: return launch_mfma<2, 2, 4, 2>(h, P);
+   case 3 * 16 + 0: return launch_mfma_pipe<3, 1, 2>(h, P);
+   case 3 * 16 + 1: return launch_mfma_pipe<3, 2, 2>(h, P);
+   case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 3>(h, P);
+   case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
+   case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
+   case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
+   case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
+   case 6 * 16 + 3: return launch_mfma_pipe<6, 1, 2>(h, P);
    default: return hipErrorInvalidValue;
    }
 }
@@ -243,7 +267,8 @@ hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel)
    const int nd = P.nd;
    const int nt = nd <= 16 ? 1 : nd <= 32 ? 2 : nd <= 48 ? 3 : 6;
    if (kernel >= 2 && kernel < 2 + N_MFMA_VARIANTS) { return launch_mfma_variant(h, P, nt, kernel - 2); }
-   if (kernel == 0 && nd >= 45) { return launch_mfma_variant(h, P, nt, h->mfma_variant); }
+   if (kernel == 0 && h->local_kernel != 0) { return launch_local(h, P, h->local_kernel); }
+   if (kernel == 0 && nd >= 45) { return launch_mfma_variant(h, P, nt, 0); }
    const int n2 = nd * nd;
    if (n2 <= 256) { return launch_generic<1, 256>(h, P); }
    if (n2 <= 512) { return launch_generic<2, 256>(h, P); }
@@ -454,6 +479,15 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
    std::vector<unsigned char> ess((size_t)n, 0);
    if (is_ess) { std::memcpy(ess.data(), is_ess, (size_t)n); }
    LF_HIP(upload(h, h->ess, ess.data(), ess.size()));
+   // per (element, local dof) flags: bit0 sign < 0, bit1 essential dof
+   std::vector<double> sign(ne * nd);
+   LF_HIP(hipMemcpy(sign.data(), h->sign, sign.size() * sizeof(double), hipMemcpyDeviceToHost));
+   std::vector<unsigned char> eflag(ne * nd);
+   for (size_t i = 0; i < ne * nd; ++i)
+   {
+      eflag[i] = (unsigned char)((sign[i] < 0.0 ? 1 : 0) | (ess[(size_t)dof[i]] ? 2 : 0));
+   }
+   LF_HIP(upload(h, h->eflag, eflag.data(), eflag.size()));
    LF_HIP(hipMalloc((void **)&h->vals, (size_t)nnz * sizeof(double2)));
    LF_HIP(hipMalloc((void **)&h->slot, ne * nd * nd * sizeof(uint16_t)));
    LF_HIP(hipMalloc((void **)&h->bad, sizeof(int)));
@@ -479,6 +513,10 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
                                int kernel)
 {
    if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   if (kernel < 0 || kernel >= 2 + N_MFMA_VARIANTS)
+   {
+      return h->fail(LFASM_E_ARG, "kernel selector %d outside [0, %d)", kernel, 2 + N_MFMA_VARIANTS);
+   }
    LF_HIP(hipSetDevice(h->device));
    const LocalParams P = local_params(h, mask_cc, mask_mass, apply_te, h->Ae);
    LF_HIP(ev_begin(h, T_LOCAL));
@@ -495,6 +533,17 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
    LF_HIP(ev_end(h, T_LOCAL));
    LF_HIP(hipStreamSynchronize(h->stream));
    h->local_valid = true;
+   return LFASM_OK;
+}
+
+int lfasm_set_local_kernel(lfasm_handle *h, int kernel)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (kernel < 0 || kernel >= 2 + N_MFMA_VARIANTS)
+   {
+      return h->fail(LFASM_E_ARG, "kernel selector %d outside [0, %d)", kernel, 2 + N_MFMA_VARIANTS);
+   }
+   h->local_kernel = kernel;
    return LFASM_OK;
 }
 
@@ -515,8 +564,7 @@ int lfasm_scatter_eliminate(lfasm_handle *h, int eliminate)
    S.nd = h->nd;
    S.Ip = h->Ip;
    S.Jc = h->Jc;
-   S.dof = h->dof;
-   S.sign = h->sign;
+   S.eflag = h->eflag;
    S.rc_ptr = h->rc_ptr;
    S.rc_list = h->rc_list;
    S.slot = h->slot;
@@ -703,5 +751,71 @@ int64_t lfasm_csr_symbolic(int ne, int nd, int height, const int *dof, int *I, i
 }
 
 void lfasm_free(void *p) { std::free(p); }
+
+int lfasm_measure_peaks(int device, double *out)
+{
+   if (!out) { return LFASM_E_ARG; }
+   if (hipSetDevice(device) != hipSuccess) { return LFASM_E_HIP; }
+   hipDeviceProp_t prop;
+   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { return LFASM_E_HIP; }
+   const int ncu = prop.multiProcessorCount;
+   hipStream_t s = nullptr;
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   double *dout = nullptr;
+   double2 *a = nullptr, *b = nullptr;
+   const size_t nbytes = (size_t)1 << 31;   // 2 GiB per buffer
+   const size_t n2 = nbytes / sizeof(double2);
+   int rc = LFASM_OK;
+   float ms = 0.0f;
+   auto ok = [&](hipError_t e) { if (e != hipSuccess && rc == LFASM_OK) { rc = LFASM_E_HIP; } return rc == LFASM_OK; };
+   if (!ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) || !ok(hipEventCreate(&e0)) ||
+       !ok(hipEventCreate(&e1)) || !ok(hipMalloc((void **)&dout, 65536 * sizeof(double))) ||
+       !ok(hipMalloc((void **)&a, nbytes)) || !ok(hipMalloc((void **)&b, nbytes)))
+   {
+      goto done;
+   }
+   {
+      // FP64 MFMA: 8 waves per CU x 4 blocks per CU
+      const int iters = 4000;
+      const int grid = ncu * 8;
+      hipLaunchKernelGGL(k_peak_mfma, dim3(grid), dim3(256), 0, s, 10, dout);
+      ok(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_peak_mfma, dim3(grid), dim3(256), 0, s, iters, dout);
+      ok(hipEventRecord(e1, s));
+      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
+      const double flops = (double)grid * 4 * iters * 8 * (2.0 * 16 * 16 * 4);
+      out[0] = flops / (ms * 1e-3) / 1e12;
+   }
+   {
+      const int iters = 20000;
+      const int grid = ncu * 8;
+      hipLaunchKernelGGL(k_peak_valu, dim3(grid), dim3(256), 0, s, 10, 0.999999, dout);
+      ok(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_peak_valu, dim3(grid), dim3(256), 0, s, iters, 0.999999, dout);
+      ok(hipEventRecord(e1, s));
+      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
+      const double flops = (double)grid * 256 * iters * 8 * 2.0;
+      out[1] = flops / (ms * 1e-3) / 1e12;
+   }
+   {
+      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(256), 0, s, a, n2);
+      hipLaunchKernelGGL(k_peak_copy, dim3(ncu * 16), dim3(256), 0, s, a, b, n2);
+      ok(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_peak_copy, dim3(ncu * 16), dim3(256), 0, s, a, b, n2);
+      ok(hipEventRecord(e1, s));
+      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
+      out[2] = 2.0 * (double)nbytes / (ms * 1e-3) / 1e9;
+      out[3] = (double)prop.clockRate / 1e6;   // GHz (reported maximum)
+      out[4] = ncu;
+   }
+done:
+   if (a) { (void)hipFree(a); }
+   if (b) { (void)hipFree(b); }
+   if (dout) { (void)hipFree(dout); }
+   if (e0) { (void)hipEventDestroy(e0); }
+   if (e1) { (void)hipEventDestroy(e1); }
+   if (s) { (void)hipStreamDestroy(s); }
+   return rc;
+}
 
 }  // extern "C"

@@ -349,76 +349,14 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const double *e
 // has 64 WPE EPB threads.  WAVES_PER_EU bounds the register budget.
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2v __attribute__((ext_vector_type(2)));
 
-template <int NT, int RT, int EPB, int WAVES_PER_EU>
-__global__ void __launch_bounds__(64 * (NT / RT) * EPB)
-__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
-k_local_mfma(LocalParams P, int nqt)
+// Face transform and store of the MFMA accumulators of one wavefront
+// (row tiles wsub*RT .. wsub*RT+RT-1, all NT column tiles).
+template <int NT, int RT>
+__device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[RT][NT][2],
+                                              int e, bool live, int wsub, int g4, int m16)
 {
-   constexpr int NP = 16 * NT;
-   constexpr int WPE = NT / RT;
-   static_assert(NT % RT == 0, "row tiles must divide the tile count");
-   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-   const int lane = threadIdx.x & 63;
-   const int slot = wave / WPE, wsub = wave - slot * WPE;
-   const int e_raw = blockIdx.x * EPB + slot;
-   const bool live = e_raw < P.ne;
-   const int e = live ? e_raw : P.ne - 1;
-
-   extern __shared__ double smem[];
-   const int slot_doubles = EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP;
-   double *ec = smem + (size_t)slot * slot_doubles;
-   double *qcoef = ec + EC_SIZE;
-   double *A = qcoef + nqt * 4;
-   double *Cr = A + CHUNK_ROWS * NP;
-   double *Ci = Cr + CHUNK_ROWS * NP;
-   const int gtid = wsub * 64 + lane;   // thread index within the element group
-
-   elem_setup(P, e, nqt, gtid, WPE * 64, qcoef, ec);
-   const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
-   __syncthreads();
-
-   dbl4 acc[RT][NT][2];
-#pragma unroll
-   for (int t = 0; t < RT; ++t)
-#pragma unroll
-      for (int j = 0; j < NT; ++j)
-      {
-         acc[t][j][0] = dbl4{0.0, 0.0, 0.0, 0.0};
-         acc[t][j][1] = dbl4{0.0, 0.0, 0.0, 0.0};
-      }
-
-   const int g4 = lane >> 4, m16 = lane & 15;
-   const int nch = ss.nch_cc + ss.nch_m;
-   for (int ch = 0; ch < nch; ++ch)
-   {
-      form_chunk(P, ec, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
-      __syncthreads();
-#pragma unroll 1
-      for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
-      {
-         const int row = ks * 4 + g4;
-         double a[RT], br[NT], bi[NT];
-#pragma unroll
-         for (int t = 0; t < RT; ++t) { a[t] = A[row * NP + 16 * (wsub * RT + t) + m16]; }
-#pragma unroll
-         for (int j = 0; j < NT; ++j)
-         {
-            br[j] = Cr[row * NP + 16 * j + m16];
-            bi[j] = Ci[row * NP + 16 * j + m16];
-         }
-#pragma unroll
-         for (int t = 0; t < RT; ++t)
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
-            {
-               acc[t][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], br[j], acc[t][j][0], 0, 0, 0);
-               acc[t][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bi[j], acc[t][j][1], 0, 0, 0);
-            }
-      }
-      __syncthreads();
-   }
-
    // dual face-DOF transform TInv(fo)^T A TInv: columns, then rows
    // (solver/asm_gpu.inc:221-251)
    const int nd = P.nd, p = P.p;
@@ -496,6 +434,158 @@ k_local_mfma(LocalParams P, int nqt)
             }
          }
       }
+}
+
+template <int NT, int RT, int EPB, int WAVES_PER_EU>
+__global__ void __launch_bounds__(64 * (NT / RT) * EPB)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+k_local_mfma(LocalParams P, int nqt)
+{
+   constexpr int NP = 16 * NT;
+   constexpr int WPE = NT / RT;
+   static_assert(NT % RT == 0, "row tiles must divide the tile count");
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int slot = wave / WPE, wsub = wave - slot * WPE;
+   const int e_raw = blockIdx.x * EPB + slot;
+   const bool live = e_raw < P.ne;
+   const int e = live ? e_raw : P.ne - 1;
+
+   extern __shared__ double smem[];
+   const int slot_doubles = EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP;
+   double *ec = smem + (size_t)slot * slot_doubles;
+   double *qcoef = ec + EC_SIZE;
+   double *A = qcoef + nqt * 4;
+   double *Cr = A + CHUNK_ROWS * NP;
+   double *Ci = Cr + CHUNK_ROWS * NP;
+   const int gtid = wsub * 64 + lane;   // thread index within the element group
+
+   elem_setup(P, e, nqt, gtid, WPE * 64, qcoef, ec);
+   const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
+   __syncthreads();
+
+   dbl4 acc[RT][NT][2];
+#pragma unroll
+   for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+      {
+         acc[t][j][0] = dbl4{0.0, 0.0, 0.0, 0.0};
+         acc[t][j][1] = dbl4{0.0, 0.0, 0.0, 0.0};
+      }
+
+   const int g4 = lane >> 4, m16 = lane & 15;
+   const int nch = ss.nch_cc + ss.nch_m;
+   for (int ch = 0; ch < nch; ++ch)
+   {
+      form_chunk(P, ec, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
+      __syncthreads();
+#pragma unroll 1
+      for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
+      {
+         const int row = ks * 4 + g4;
+         double a[RT], br[NT], bi[NT];
+#pragma unroll
+         for (int t = 0; t < RT; ++t) { a[t] = A[row * NP + 16 * (wsub * RT + t) + m16]; }
+#pragma unroll
+         for (int j = 0; j < NT; ++j)
+         {
+            br[j] = Cr[row * NP + 16 * j + m16];
+            bi[j] = Ci[row * NP + 16 * j + m16];
+         }
+#pragma unroll
+         for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+            {
+               acc[t][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], br[j], acc[t][j][0], 0, 0, 0);
+               acc[t][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bi[j], acc[t][j][1], 0, 0, 0);
+            }
+      }
+      __syncthreads();
+   }
+
+   mfma_epilogue<NT, RT>(P, acc, e, live, wsub, g4, m16);
+}
+
+// Pipelined variant: one row tile per wavefront (NT wavefronts per
+// element), one (point, basis) item per thread and chunk, double-buffered
+// staging: the MFMAs of chunk c and the forming of chunk c+1 share one
+// barrier interval, so table loads and coefficient arithmetic overlap the
+// matrix-core work.
+template <int NT, int EPB, int WAVES_PER_EU>
+__global__ void __launch_bounds__(64 * NT * EPB)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+k_local_mfma_pipe(LocalParams P, int nqt)
+{
+   constexpr int NP = 16 * NT;
+   constexpr int STG = 3 * CHUNK_ROWS * NP;     // one staging buffer (A, Cr, Ci)
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int slot = wave / NT, wsub = wave - slot * NT;
+   const int e_raw = blockIdx.x * EPB + slot;
+   const bool live = e_raw < P.ne;
+   const int e = live ? e_raw : P.ne - 1;
+
+   extern __shared__ double smem[];
+   const int slot_doubles = EC_SIZE + nqt * 4 + 2 * STG;
+   double *ec = smem + (size_t)slot * slot_doubles;
+   double *qcoef = ec + EC_SIZE;
+   double *buf = qcoef + nqt * 4;
+   const int gtid = wsub * 64 + lane;
+
+   elem_setup(P, e, nqt, gtid, NT * 64, qcoef, ec);
+   const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
+   __syncthreads();
+
+   dbl4 acc[1][NT][2];
+#pragma unroll
+   for (int j = 0; j < NT; ++j)
+   {
+      acc[0][j][0] = dbl4{0.0, 0.0, 0.0, 0.0};
+      acc[0][j][1] = dbl4{0.0, 0.0, 0.0, 0.0};
+   }
+   const int g4 = lane >> 4, m16 = lane & 15;
+   const int nch = ss.nch_cc + ss.nch_m;
+   if (nch > 0)
+   {
+      form_chunk(P, ec, ss, qcoef, 0, gtid, NT * 64, buf, buf + CHUNK_ROWS * NP,
+                 buf + 2 * CHUNK_ROWS * NP, NP);
+   }
+   __syncthreads();
+   for (int ch = 0; ch < nch; ++ch)
+   {
+      const double *A = buf + (ch & 1) * STG;
+      const double *Cr = A + CHUNK_ROWS * NP;
+      const double *Ci = Cr + CHUNK_ROWS * NP;
+#pragma unroll
+      for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
+      {
+         const int row = ks * 4 + g4;
+         const double a = A[row * NP + 16 * wsub + m16];
+         double br[NT], bi[NT];
+#pragma unroll
+         for (int j = 0; j < NT; ++j)
+         {
+            br[j] = Cr[row * NP + 16 * j + m16];
+            bi[j] = Ci[row * NP + 16 * j + m16];
+         }
+#pragma unroll
+         for (int j = 0; j < NT; ++j)
+         {
+            acc[0][j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, br[j], acc[0][j][0], 0, 0, 0);
+            acc[0][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bi[j], acc[0][j][1], 0, 0, 0);
+         }
+      }
+      if (ch + 1 < nch)
+      {
+         double *An = buf + ((ch + 1) & 1) * STG;
+         form_chunk(P, ec, ss, qcoef, ch + 1, gtid, NT * 64, An, An + CHUNK_ROWS * NP,
+                    An + 2 * CHUNK_ROWS * NP, NP);
+      }
+      __syncthreads();
+   }
+   mfma_epilogue<NT, 1>(P, acc, e, live, wsub, g4, m16);
 }
 
 // ---- generic VALU local-matrix kernel (any nd) -----------------------------
@@ -631,8 +721,7 @@ __global__ void k_slot_map(int ne, int nd, const int *__restrict__ dof,
 struct ScatterParams {
    int n, nd;
    const int *Ip, *Jc;
-   const int *dof;
-   const double *sign;
+   const unsigned char *eflag;  // per (e, b): bit0 sign < 0, bit1 dof essential
    const int *rc_ptr, *rc_list;
    const uint16_t *slot;
    const unsigned char *is_ess;
@@ -641,6 +730,18 @@ struct ScatterParams {
    int eliminate;
    int maxrow;
 };
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Contributors are processed in groups of CG: the loads of a group are
+// issued together, the LDS accumulation then runs contributor by
+// contributor (ascending element order) so the sums are deterministic.
+constexpr int SCATTER_CG = 4;
 
 __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
 {
@@ -652,34 +753,55 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
    const int base = S.Ip[r];
    const int len = S.Ip[r + 1] - base;
    for (int s = lane; s < len; s += 64) { buf[s] = make_double2(0.0, 0.0); }
-   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-   __builtin_amdgcn_wave_barrier();
-   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+   wave_sync_lds();
    const bool row_ess = S.eliminate && S.is_ess[r];
    const int nd = S.nd;
+   const unsigned char ess_bit = S.eliminate ? 2 : 0;
    if (!row_ess)
    {
       const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
-      for (int c = c0; c < c1; ++c)
+      for (int cg = c0; cg < c1; cg += SCATTER_CG)
       {
-         const int ea = S.rc_list[c];          // e*nd + a
-         const int e = ea / nd;
-         const double sa = S.sign[ea];
-         for (int b = lane; b < nd; b += 64)
+         for (int b0 = 0; b0 < nd; b0 += 64)
          {
-            const int eb = e * nd + b;
-            if (S.eliminate && S.is_ess[S.dof[eb]]) { continue; }
-            const int s = S.slot[(size_t)ea * nd + b];
-            const double sg = sa * S.sign[eb];
-            const double2 v = S.Ae[(size_t)ea * nd + b];
-            double2 t = buf[s];
-            t.x += sg * v.x;
-            t.y += sg * v.y;
-            buf[s] = t;
+            const int b = b0 + lane;
+            double2 v[SCATTER_CG];
+            int sl[SCATTER_CG];
+            bool use[SCATTER_CG];
+#pragma unroll
+            for (int u = 0; u < SCATTER_CG; ++u)
+            {
+               const int c = cg + u;
+               use[u] = false;
+               v[u] = make_double2(0.0, 0.0);
+               sl[u] = 0;
+               if (c < c1 && b < nd)
+               {
+                  const int ea = S.rc_list[c];          // e*nd + a
+                  const int e = ea / nd;
+                  const unsigned char fa = S.eflag[ea];
+                  const unsigned char fb = S.eflag[(size_t)e * nd + b];
+                  use[u] = (fb & ess_bit) == 0;
+                  sl[u] = S.slot[(size_t)ea * nd + b];
+                  const dbl2v x = __builtin_nontemporal_load(
+                     reinterpret_cast<const dbl2v *>(S.Ae) + (size_t)ea * nd + b);
+                  const bool neg = ((fa ^ fb) & 1) != 0;
+                  v[u] = neg ? make_double2(-x.x, -x.y) : make_double2(x.x, x.y);
+               }
+            }
+#pragma unroll
+            for (int u = 0; u < SCATTER_CG; ++u)
+            {
+               if (use[u])
+               {
+                  double2 t = buf[sl[u]];
+                  t.x += v[u].x;
+                  t.y += v[u].y;
+                  buf[sl[u]] = t;
+               }
+               wave_sync_lds();
+            }
          }
-         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-         __builtin_amdgcn_wave_barrier();
-         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
    }
    else
@@ -688,9 +810,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
       {
          if (S.Jc[base + s] == r) { buf[s] = make_double2(1.0, 0.0); }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_sync_lds();
    }
    for (int s = lane; s < len; s += 64) { S.vals[base + s] = buf[s]; }
 }

@@ -1,0 +1,61 @@
+// On-device peak probes: FP64 MFMA throughput (v_mfma_f64_16x16x4f64),
+// FP64 VALU FMA throughput, and HBM copy bandwidth.  Used to report
+// measured ceilings next to the published ones.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lfasm {
+
+typedef double pdbl4 __attribute__((ext_vector_type(4)));
+
+// Each wavefront runs `iters` rounds of 8 independent MFMA chains.
+__global__ void __launch_bounds__(256) k_peak_mfma(int iters, double *out)
+{
+   const int lane = threadIdx.x & 63;
+   const double a = 1.0 + 1e-9 * lane, b = 1.0 - 1e-9 * lane;
+   pdbl4 acc[8];
+#pragma unroll
+   for (int i = 0; i < 8; ++i) { acc[i] = pdbl4{0.0, 0.0, 0.0, 0.0}; }
+   for (int it = 0; it < iters; ++it)
+   {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+      {
+         acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+      }
+   }
+   double s = 0.0;
+#pragma unroll
+   for (int i = 0; i < 8; ++i) { s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3]; }
+   if (s == -1.0) { out[blockIdx.x] = s; }   // never true; keeps the work live
+}
+
+// Each thread runs `iters` rounds of 8 independent FMA chains.
+__global__ void __launch_bounds__(256) k_peak_valu(int iters, double y, double *out)
+{
+   double x[8];
+#pragma unroll
+   for (int i = 0; i < 8; ++i) { x[i] = 1.0 + i * 1e-3 + threadIdx.x * 1e-9; }
+   for (int it = 0; it < iters; ++it)
+   {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) { x[i] = __fma_rn(x[i], y, 1e-12); }
+   }
+   double s = 0.0;
+#pragma unroll
+   for (int i = 0; i < 8; ++i) { s += x[i]; }
+   if (s == -1.0) { out[blockIdx.x] = s; }
+}
+
+__global__ void __launch_bounds__(256) k_peak_copy(const double2 *__restrict__ src,
+                                                   double2 *__restrict__ dst, size_t n)
+{
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+        i += (size_t)gridDim.x * blockDim.x)
+   {
+      dst[i] = src[i];
+   }
+}
+
+}  // namespace lfasm

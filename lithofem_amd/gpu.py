@@ -89,6 +89,7 @@ def lib():
         "lfasm_build_global": ([vp, vp, vp, i64, vp], i),
         "lfasm_compute_local": ([vp, i, i, i], i),
         "lfasm_compute_local_kernel": ([vp, i, i, i, i], i),
+        "lfasm_set_local_kernel": ([vp, i], i),
         "lfasm_scatter_eliminate": ([vp, i], i),
         "lfasm_device_csr": ([vp, vp, vp, vp], i),
         "lfasm_download_values": ([vp, vp, vp], i),
@@ -99,6 +100,7 @@ def lib():
         "lfasm_stream": ([vp], vp),
         "lfasm_csr_symbolic": ([i, i, i, vp, vp, ct.POINTER(vp)], i64),
         "lfasm_free": ([vp], None),
+        "lfasm_measure_peaks": ([i, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -124,6 +126,15 @@ def csr_symbolic(dof: np.ndarray, height: int):
     J = np.ctypeslib.as_array(ct.cast(pj, ct.POINTER(ct.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
     lib().lfasm_free(pj)
     return I, J
+
+
+def measure_peaks(device: int = 0) -> dict:
+    out = np.zeros(5)
+    rc = lib().lfasm_measure_peaks(device, _p(out))
+    if rc != 0:
+        raise AsmGpuError(rc, "measure_peaks failed")
+    return dict(fp64_mfma_tflops=out[0], fp64_valu_tflops=out[1], hbm_copy_gbps=out[2],
+                clock_ghz=out[3], cus=int(out[4]))
 
 
 class AsmGpuGlobal:
@@ -218,6 +229,9 @@ class AsmGpuGlobal:
                       kernel=KERNEL_AUTO):
         self._check(lib().lfasm_compute_local_kernel(self._h, mask_cc, mask_mass,
                                                      1 if apply_te else 0, kernel))
+
+    def set_local_kernel(self, kernel: int):
+        self._check(lib().lfasm_set_local_kernel(self._h, kernel))
 
     # -- AsmGpuGlobal::ScatterEliminate
     def scatter_eliminate(self, eliminate=True):
