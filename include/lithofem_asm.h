@@ -104,6 +104,18 @@ double lfasm_pml_smax(double target, double order, double k0, double n_med, doub
  * LITHOFEM_ASM_FORCE_FAIL (returns LFASM_E_FORCED before any device call). */
 int lfasm_init(lfasm_handle *h, const lfasm_mesh *m);
 
+/* Batched multi-mesh init (no reference counterpart): the meshes (same
+ * order, same rule tables and TInv; each with its own geometry, region
+ * table, PML and k_par) are assembled as one block-diagonal system whose
+ * elements and dofs are the meshes' in order.  build_global then takes the
+ * union pattern (each mesh's CSR with rows/columns offset by the preceding
+ * meshes' dof counts); every kernel launch covers all meshes at once. */
+int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *meshes, int nmesh);
+
+/* Element and dof offsets of the batched meshes (nmesh+1 entries each);
+ * returns nmesh. */
+int lfasm_batch_offsets(const lfasm_handle *h, int64_t *elem_off, int64_t *row_off);
+
 /* Per-element complex eps (no reference counterpart: the reference reads
  * region eps only, solver/asm_gpu.inc:96).  eps = ne interleaved complex, or
  * NULL to return to the region table. */
@@ -168,6 +180,52 @@ int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
  * [0] compute_local kernels, [1] scatter_eliminate kernels,
  * [2] build_global device part, [3] last batched call, [4] last adjoint call. */
 int lfasm_timings(const lfasm_handle *h, double *ms, int n);
+
+/* ---- eps variants (no reference counterpart) ------------------------------
+ * Many per-element eps sets on one mesh by delta assembly: the mass term is
+ * linear in eps, so A(eps_v) = A_base + sum_e deps_{v,e} Mhat_e with Mhat_e
+ * the face-transformed unit-eps mass matrix of parameter element e.
+ * prepare: parameter elements (sorted, unique element ids); computes Mhat.
+ * assemble: A_base = the handle's current CSR values (compute_local +
+ * scatter_eliminate with the same `eliminate`); deps = nvar x nparam
+ * interleaved complex; writes nvar value arrays on the device. */
+int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems);
+int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int eliminate);
+int lfasm_variant_values(lfasm_handle *h, int v, void **dev_vals);
+int lfasm_download_variant(lfasm_handle *h, int v, double *re, double *im);
+
+/* ---- incident fields, batched RHS and eps sensitivity ----------------------
+ * Incident envelope per slab (the reference's SlabWave,
+ * solver/problem_common.inc:11-16,218-236): E(z) = A exp(i qA (z-zA)) +
+ * B exp(i qB (z-zB)).  waves: [problem][rhs][slab] x 18 doubles
+ * (zA, zB, qA re/im, qB re/im, A[3] re/im, B[3] re/im).  Region arrays are
+ * concatenated per problem like the eps tables. */
+typedef struct {
+   int nrhs;                  /* incident fields per problem */
+   int nslab;
+   const double *waves;
+   const int *region_slab;    /* slab index of each region */
+   const double *region_deps; /* interleaved complex eps - eps_bg (source strength) */
+   const int *region_src;     /* 1: scattered-field source region (frustum) */
+} lfasm_incident;
+
+int lfasm_set_incident(lfasm_handle *h, const lfasm_incident *inc);
+
+/* RHS of every incident field (ScatterSource, solver/problem_common.inc:
+ * 238-266): b_k = int k0^2 deps E(z) . v_k on the mass rule, dual face
+ * transform, sign scatter (ascending element order); essential rows zeroed
+ * with `eliminate` (solver/lithofem_solve.cpp:732-742).  Device result
+ * nrhs x n complex. */
+int lfasm_assemble_rhs(lfasm_handle *h, int eliminate);
+int lfasm_download_rhs(lfasm_handle *h, double *re, double *im);
+
+/* Adjoint sensitivity per parameter element (sorted unique element ids):
+ * s_e = lam^T (dA/deps_e u - db/deps_e) with lam, u global complex vectors
+ * (n interleaved, host); the RHS term uses incident field rhs_col of the
+ * element's problem in source regions (rhs_col < 0: matrix term only).
+ * s_out: nparam interleaved complex. */
+int lfasm_eps_sensitivity(lfasm_handle *h, int nparam, const int *elems, const double *lam,
+                          const double *u, int rhs_col, double *s_out);
 
 /* The handle's HIP stream (hipStream_t) for callers that record their own
  * events or order their own work after the assembly. */

@@ -6,6 +6,7 @@
 // message string instead of printf + bool (solver/asm_gpu.inc:255-263).
 #include "lfasm_device.hpp"
 #include "lfasm_peaks.hpp"
+#include "lfasm_batch.hpp"
 #include "../../include/lithofem_asm.h"
 
 #include <hip/hip_runtime.h>
@@ -44,7 +45,11 @@ struct lfasm_handle {
    double *sign = nullptr;
    double2 *eps_tab = nullptr, *eps_elem = nullptr;
    DevRule rc, rm;
-   PmlDev pml{};
+   PmlDev *pml_tab = nullptr;
+   int *pid = nullptr, *eps_off = nullptr;
+   bool any_kpar = false;
+   int nmesh = 0;
+   std::vector<int64_t> elem_off, row_off;
    double TInv[24];
    double *TInv_dev = nullptr;
    double2 *Ae = nullptr;
@@ -55,6 +60,17 @@ struct lfasm_handle {
    uint16_t *slot = nullptr;
    unsigned char *ess = nullptr, *eflag = nullptr;
    double2 *vals = nullptr;
+   // eps variants
+   int nparam = 0, nvar = 0;
+   int *pelem = nullptr, *prc_ptr = nullptr, *prc_list = nullptr;
+   double2 *Mhat = nullptr, *var_vals = nullptr;
+   std::vector<int> pelem_host;
+   // incident fields / RHS
+   int inc_nrhs = 0, inc_nslab = 0;
+   SlabWaveDev *waves = nullptr;
+   int *region_slab = nullptr, *region_src = nullptr;
+   std::vector<int> region_src_host;
+   double2 *region_deps = nullptr, *rhs = nullptr;
    // timing
    hipEvent_t ev[2 * T_N] = {};
    bool ev_used[T_N] = {};
@@ -106,6 +122,9 @@ void release_mesh(lfasm_handle *h)
    dfree(h->J); dfree(h->Ji); dfree(h->detJ); dfree(h->v0);
    dfree(h->attr); dfree(h->fo); dfree(h->dof); dfree(h->sign);
    dfree(h->eps_tab); dfree(h->eps_elem); dfree(h->TInv_dev); dfree(h->Ae);
+   dfree(h->pml_tab); dfree(h->pid); dfree(h->eps_off);
+   dfree(h->waves); dfree(h->region_slab); dfree(h->region_src); dfree(h->region_deps);
+   h->inc_nrhs = 0;
    for (DevRule *r : {&h->rc, &h->rm})
    {
       dfree(r->ip); dfree(r->w); dfree(r->vs); dfree(r->cs);
@@ -119,6 +138,9 @@ void release_global(lfasm_handle *h)
 {
    dfree(h->Ip); dfree(h->Jc); dfree(h->rc_ptr); dfree(h->rc_list);
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
+   dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
+   dfree(h->rhs);
+   h->nparam = h->nvar = 0;
    h->global_built = false;
 }
 
@@ -158,13 +180,14 @@ LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_
    P.fo = h->fo;
    P.rc = RuleDev{h->rc.nq, h->rc.ip, h->rc.w, h->rc.vs, h->rc.cs};
    P.rm = RuleDev{h->rm.nq, h->rm.ip, h->rm.w, h->rm.vs, h->rm.cs};
-   P.pml = h->pml;
+   P.pml_tab = h->pml_tab;
+   P.pid = h->pid;
+   P.eps_off = h->eps_off;
    std::memcpy(P.TInv, h->TInv, sizeof(P.TInv));
    P.use_cc = (mask_cc & LFASM_CURLCURL) ? 1 : 0;
    int mf = 0;
    if (mask_mass & LFASM_MASS) { mf |= 1; }
-   const bool kpar = h->pml.kx != 0.0 || h->pml.ky != 0.0;
-   if (kpar)
+   if (h->any_kpar)
    {
       // the k-parallel integrators exist only when k_par != 0
       // (solver/lithofem_solve.cpp:695-704)
@@ -175,6 +198,9 @@ LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_
    P.mflags = mf;
    P.apply_te = apply_te && h->p >= 2;
    P.Ae = Ae;
+   P.nout = h->ne;
+   P.elist = nullptr;
+   P.eps_unit = 0;
    return P;
 }
 
@@ -189,7 +215,7 @@ hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma<NT, RT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   const int grid = (P.ne + EPB - 1) / EPB;
+   const int grid = (P.nout + EPB - 1) / EPB;
    hipLaunchKernelGGL((k_local_mfma<NT, RT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
                       h->stream, P, nqt);
    return hipGetLastError();
@@ -204,12 +230,12 @@ hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_generic<NACC, NTHR>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   hipLaunchKernelGGL((k_local_generic<NACC, NTHR>), dim3(P.ne), dim3(NTHR), shmem, h->stream,
+   hipLaunchKernelGGL((k_local_generic<NACC, NTHR>), dim3(P.nout), dim3(NTHR), shmem, h->stream,
                       P, nqt, NP);
    e = hipGetLastError();
    if (e != hipSuccess || !P.apply_te) { return e; }
-   hipLaunchKernelGGL(k_te, dim3(P.ne), dim3(64), 0, h->stream, P.nd, P.p, P.fo, h->TInv_dev,
-                      P.Ae);
+   hipLaunchKernelGGL(k_te, dim3(P.nout), dim3(64), 0, h->stream, P.nd, P.p, P.fo, P.elist,
+                      h->TInv_dev, P.Ae);
    return hipGetLastError();
 }
 
@@ -224,7 +250,7 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_pipe<NT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   const int grid = (P.ne + EPB - 1) / EPB;
+   const int grid = (P.nout + EPB - 1) / EPB;
    hipLaunchKernelGGL((k_local_mfma_pipe<NT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
                       h->stream, P, nqt);
    return hipGetLastError();
@@ -337,27 +363,25 @@ double lfasm_pml_smax(double target, double order, double k0, double n_med, doub
    return -std::log(target) * (order + 1) / (2.0 * k0 * n_med * thick);
 }
 
-int lfasm_init(lfasm_handle *h, const lfasm_mesh *m)
+}  // extern "C"
+
+namespace {
+
+int validate_mesh(lfasm_handle *h, const lfasm_mesh *m, int idx)
 {
-   if (!h || !h->stream) { return LFASM_E_STATE; }
-   if (std::getenv("LITHOFEM_ASM_FORCE_FAIL"))
-   {
-      return h->fail(LFASM_E_FORCED, "forced failure injected (LITHOFEM_ASM_FORCE_FAIL)");
-   }
-   if (!m) { return h->fail(LFASM_E_ARG, "null mesh"); }
    if (m->ne <= 0 || m->nd <= 0 || m->p <= 0 || m->height <= 0 || m->nreg <= 0)
    {
-      return h->fail(LFASM_E_ARG, "ne, nd, p, height, nreg must be positive");
+      return h->fail(LFASM_E_ARG, "mesh %d: ne, nd, p, height, nreg must be positive", idx);
    }
    if (m->p > 4 || m->nd != m->p * (m->p + 2) * (m->p + 3) / 2)
    {
-      return h->fail(LFASM_E_UNSUPPORTED, "order %d / nd %d outside the kernels (p <= 4)",
-                     m->p, m->nd);
+      return h->fail(LFASM_E_UNSUPPORTED, "mesh %d: order %d / nd %d outside the kernels (p <= 4)",
+                     idx, m->p, m->nd);
    }
    if (!m->dof || !m->sign || !m->attr || !m->J || !m->Jinv || !m->detJ || !m->v0 ||
        !m->eps_tab || !m->TInv || (m->p >= 2 && !m->fo))
    {
-      return h->fail(LFASM_E_ARG, "missing mesh array");
+      return h->fail(LFASM_E_ARG, "mesh %d: missing mesh array", idx);
    }
    int rc;
    if ((rc = check_rule(h, m->rule_cc, "cc")) != LFASM_OK) { return rc; }
@@ -367,18 +391,21 @@ int lfasm_init(lfasm_handle *h, const lfasm_mesh *m)
    {
       if (m->dof[i] < 0 || m->dof[i] >= m->height)
       {
-         return h->fail(LFASM_E_ARG, "dof[%zu] = %d outside [0, %d)", i, m->dof[i], m->height);
+         return h->fail(LFASM_E_ARG, "mesh %d: dof[%zu] = %d outside [0, %d)", idx, i, m->dof[i],
+                        m->height);
       }
    }
    for (int e = 0; e < m->ne; ++e)
    {
       if (m->attr[e] < 1 || m->attr[e] > m->nreg)
       {
-         return h->fail(LFASM_E_ARG, "attr[%d] = %d outside [1, %d]", e, m->attr[e], m->nreg);
+         return h->fail(LFASM_E_ARG, "mesh %d: attr[%d] = %d outside [1, %d]", idx, e, m->attr[e],
+                        m->nreg);
       }
       if (!(m->detJ[e] > 0.0))
       {
-         return h->fail(LFASM_E_ARG, "detJ[%d] <= 0 (expected positively oriented tet)", e);
+         return h->fail(LFASM_E_ARG, "mesh %d: detJ[%d] <= 0 (expected positively oriented tet)",
+                        idx, e);
       }
    }
    if (m->p >= 2)
@@ -387,38 +414,166 @@ int lfasm_init(lfasm_handle *h, const lfasm_mesh *m)
       {
          if (m->fo[i] < 0 || m->fo[i] > 5)
          {
-            return h->fail(LFASM_E_ARG, "fo[%zu] = %d outside 0..5", i, m->fo[i]);
+            return h->fail(LFASM_E_ARG, "mesh %d: fo[%zu] = %d outside 0..5", idx, i, m->fo[i]);
          }
       }
+   }
+   return LFASM_OK;
+}
+
+bool same_rule(const lfasm_rule &a, const lfasm_rule &b, int nd)
+{
+   if (a.nq != b.nq) { return false; }
+   const size_t n3 = (size_t)a.nq * 3, nt = (size_t)a.nq * nd * 3;
+   return std::memcmp(a.ip, b.ip, n3 * sizeof(double)) == 0 &&
+          std::memcmp(a.w, b.w, (size_t)a.nq * sizeof(double)) == 0 &&
+          std::memcmp(a.vshape, b.vshape, nt * sizeof(double)) == 0 &&
+          std::memcmp(a.curl, b.curl, nt * sizeof(double)) == 0;
+}
+
+template <typename T>
+void append(std::vector<T> &dst, const T *src, size_t n)
+{
+   dst.insert(dst.end(), src, src + n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
+{
+   if (!h || !h->stream) { return LFASM_E_STATE; }
+   if (std::getenv("LITHOFEM_ASM_FORCE_FAIL"))
+   {
+      return h->fail(LFASM_E_FORCED, "forced failure injected (LITHOFEM_ASM_FORCE_FAIL)");
+   }
+   if (!ms || nmesh <= 0) { return h->fail(LFASM_E_ARG, "no mesh"); }
+   int rc;
+   int64_t ne_tot = 0, n_tot = 0;
+   for (int i = 0; i < nmesh; ++i)
+   {
+      if ((rc = validate_mesh(h, &ms[i], i)) != LFASM_OK) { return rc; }
+      if (ms[i].p != ms[0].p || !same_rule(ms[i].rule_cc, ms[0].rule_cc, ms[0].nd) ||
+          !same_rule(ms[i].rule_mass, ms[0].rule_mass, ms[0].nd) ||
+          std::memcmp(ms[i].TInv, ms[0].TInv, 24 * sizeof(double)) != 0)
+      {
+         return h->fail(LFASM_E_ARG, "mesh %d: batched meshes need the same order, rules and TInv", i);
+      }
+      ne_tot += ms[i].ne;
+      n_tot += ms[i].height;
+   }
+   if (ne_tot > INT32_MAX || n_tot > INT32_MAX || ne_tot * ms[0].nd > INT32_MAX)
+   {
+      return h->fail(LFASM_E_UNSUPPORTED, "batch exceeds int32 element/dof indexing");
    }
    LF_HIP(hipSetDevice(h->device));
    release_global(h);
    release_mesh(h);
-   h->ne = m->ne;
-   h->nd = m->nd;
-   h->p = m->p;
-   h->height = m->height;
-   h->nreg = m->nreg;
-   const size_t ne = (size_t)m->ne, nd = (size_t)m->nd;
-   LF_HIP(upload(h, h->J, m->J, ne * 9));
-   LF_HIP(upload(h, h->Ji, m->Jinv, ne * 9));
-   LF_HIP(upload(h, h->detJ, m->detJ, ne));
-   LF_HIP(upload(h, h->v0, m->v0, ne * 3));
-   LF_HIP(upload(h, h->attr, m->attr, ne));
-   LF_HIP(upload(h, h->dof, m->dof, ne * nd));
-   LF_HIP(upload(h, h->sign, m->sign, ne * nd));
-   if (m->p >= 2) { LF_HIP(upload(h, h->fo, m->fo, ne * 4)); }
-   LF_HIP(upload_rule(h, h->rc, m->rule_cc, m->nd));
-   LF_HIP(upload_rule(h, h->rm, m->rule_mass, m->nd));
-   LF_HIP(upload(h, h->eps_tab, (const double2 *)m->eps_tab, (size_t)m->nreg));
-   if (m->eps_elem) { LF_HIP(upload(h, h->eps_elem, (const double2 *)m->eps_elem, ne)); }
-   std::memcpy(h->TInv, m->TInv, sizeof(h->TInv));
+   const lfasm_mesh &m0 = ms[0];
+   h->ne = (int)ne_tot;
+   h->nd = m0.nd;
+   h->p = m0.p;
+   h->height = (int)n_tot;
+   h->nmesh = nmesh;
+   h->elem_off.assign(1, 0);
+   h->row_off.assign(1, 0);
+   const size_t nd = (size_t)m0.nd;
+   std::vector<PmlDev> pml;
+   std::vector<int> eps_off;
+   std::vector<double> eps_tab;
+   bool kpar = false;
+   for (int i = 0; i < nmesh; ++i)
+   {
+      const lfasm_mesh &m = ms[i];
+      pml.push_back(PmlDev{m.pml.z_min, m.pml.z_max, m.pml.thick, m.pml.order, m.pml.smax_bot,
+                           m.pml.smax_top, m.pml.k0, m.pml.kx, m.pml.ky});
+      kpar = kpar || m.pml.kx != 0.0 || m.pml.ky != 0.0;
+      eps_off.push_back((int)(eps_tab.size() / 2));
+      append(eps_tab, m.eps_tab, (size_t)m.nreg * 2);
+      h->elem_off.push_back(h->elem_off.back() + m.ne);
+      h->row_off.push_back(h->row_off.back() + m.height);
+   }
+   h->any_kpar = kpar;
+   h->nreg = (int)(eps_tab.size() / 2);
+   const size_t ne = (size_t)ne_tot;
+   if (nmesh == 1)
+   {
+      LF_HIP(upload(h, h->J, m0.J, ne * 9));
+      LF_HIP(upload(h, h->Ji, m0.Jinv, ne * 9));
+      LF_HIP(upload(h, h->detJ, m0.detJ, ne));
+      LF_HIP(upload(h, h->v0, m0.v0, ne * 3));
+      LF_HIP(upload(h, h->attr, m0.attr, ne));
+      LF_HIP(upload(h, h->dof, m0.dof, ne * nd));
+      LF_HIP(upload(h, h->sign, m0.sign, ne * nd));
+      if (m0.p >= 2) { LF_HIP(upload(h, h->fo, m0.fo, ne * 4)); }
+      if (m0.eps_elem) { LF_HIP(upload(h, h->eps_elem, (const double2 *)m0.eps_elem, ne)); }
+   }
+   else
+   {
+      // block-diagonal union: element arrays concatenated, dofs offset by
+      // the preceding meshes' dof counts, one problem id per element
+      std::vector<double> J, Ji, det, v0, sign, eel;
+      std::vector<int> attr, dof, fo, pid;
+      bool any_eel = false;
+      for (int i = 0; i < nmesh; ++i) { any_eel = any_eel || ms[i].eps_elem; }
+      for (int i = 0; i < nmesh; ++i)
+      {
+         const lfasm_mesh &m = ms[i];
+         const size_t mne = (size_t)m.ne;
+         append(J, m.J, mne * 9);
+         append(Ji, m.Jinv, mne * 9);
+         append(det, m.detJ, mne);
+         append(v0, m.v0, mne * 3);
+         append(attr, m.attr, mne);
+         append(sign, m.sign, mne * nd);
+         const int off = (int)h->row_off[(size_t)i];
+         for (size_t k = 0; k < mne * nd; ++k) { dof.push_back(m.dof[k] + off); }
+         if (m.p >= 2) { append(fo, m.fo, mne * 4); }
+         pid.insert(pid.end(), mne, i);
+         if (any_eel)
+         {
+            for (size_t e = 0; e < mne; ++e)
+            {
+               if (m.eps_elem) { append(eel, m.eps_elem + 2 * e, 2); }
+               else { append(eel, m.eps_tab + 2 * (m.attr[e] - 1), 2); }
+            }
+         }
+      }
+      LF_HIP(upload(h, h->J, J.data(), J.size()));
+      LF_HIP(upload(h, h->Ji, Ji.data(), Ji.size()));
+      LF_HIP(upload(h, h->detJ, det.data(), det.size()));
+      LF_HIP(upload(h, h->v0, v0.data(), v0.size()));
+      LF_HIP(upload(h, h->attr, attr.data(), attr.size()));
+      LF_HIP(upload(h, h->dof, dof.data(), dof.size()));
+      LF_HIP(upload(h, h->sign, sign.data(), sign.size()));
+      if (m0.p >= 2) { LF_HIP(upload(h, h->fo, fo.data(), fo.size())); }
+      LF_HIP(upload(h, h->pid, pid.data(), pid.size()));
+      if (any_eel) { LF_HIP(upload(h, h->eps_elem, (const double2 *)eel.data(), eel.size() / 2)); }
+   }
+   LF_HIP(upload_rule(h, h->rc, m0.rule_cc, m0.nd));
+   LF_HIP(upload_rule(h, h->rm, m0.rule_mass, m0.nd));
+   LF_HIP(upload(h, h->eps_tab, (const double2 *)eps_tab.data(), eps_tab.size() / 2));
+   LF_HIP(upload(h, h->eps_off, eps_off.data(), eps_off.size()));
+   LF_HIP(upload(h, h->pml_tab, pml.data(), pml.size()));
+   std::memcpy(h->TInv, m0.TInv, sizeof(h->TInv));
    LF_HIP(upload(h, h->TInv_dev, h->TInv, 24));
-   h->pml = PmlDev{m->pml.z_min, m->pml.z_max, m->pml.thick, m->pml.order,
-                   m->pml.smax_bot, m->pml.smax_top, m->pml.k0, m->pml.kx, m->pml.ky};
    LF_HIP(hipMalloc((void **)&h->Ae, ne * nd * nd * sizeof(double2)));
    h->inited = true;
    return LFASM_OK;
+}
+
+int lfasm_init(lfasm_handle *h, const lfasm_mesh *m) { return lfasm_init_batch(h, m, 1); }
+
+int lfasm_batch_offsets(const lfasm_handle *h, int64_t *elem_off, int64_t *row_off)
+{
+   if (!h || !h->inited) { return LFASM_E_STATE; }
+   for (int i = 0; i <= h->nmesh; ++i)
+   {
+      if (elem_off) { elem_off[i] = h->elem_off[(size_t)i]; }
+      if (row_off) { row_off[i] = h->row_off[(size_t)i]; }
+   }
+   return h->nmesh;
 }
 
 int lfasm_set_eps_elem(lfasm_handle *h, const double *eps)
@@ -684,6 +839,340 @@ int lfasm_timings(const lfasm_handle *hc, double *ms, int n)
       LF_HIP(hipEventElapsedTime(&f, h->ev[2 * t], h->ev[2 * t + 1]));
       ms[t] = f;
    }
+   return LFASM_OK;
+}
+
+
+// ---- eps variants ---------------------------------------------------------------
+
+int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   if (nparam <= 0 || !elems) { return h->fail(LFASM_E_ARG, "no parameter elements"); }
+   for (int i = 0; i < nparam; ++i)
+   {
+      if (elems[i] < 0 || elems[i] >= h->ne || (i > 0 && elems[i] <= elems[i - 1]))
+      {
+         return h->fail(LFASM_E_ARG, "parameter elements must be sorted, unique, in [0, ne)");
+      }
+   }
+   LF_HIP(hipSetDevice(h->device));
+   dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
+   h->nvar = 0;
+   const size_t nd = (size_t)h->nd;
+   h->pelem_host.assign(elems, elems + nparam);
+   LF_HIP(upload(h, h->pelem, elems, (size_t)nparam));
+   // per-row parameter contributors, ascending element order
+   std::vector<int> dof((size_t)h->ne * nd);
+   LF_HIP(hipMemcpy(dof.data(), h->dof, dof.size() * sizeof(int), hipMemcpyDeviceToHost));
+   std::vector<int> ptr((size_t)h->height + 1, 0), lst((size_t)nparam * nd);
+   for (int i = 0; i < nparam; ++i)
+      for (size_t a = 0; a < nd; ++a) { ptr[(size_t)dof[(size_t)elems[i] * nd + a] + 1]++; }
+   for (int r = 0; r < h->height; ++r) { ptr[(size_t)r + 1] += ptr[(size_t)r]; }
+   {
+      std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+      for (int i = 0; i < nparam; ++i)
+         for (size_t a = 0; a < nd; ++a)
+         {
+            lst[(size_t)pos[(size_t)dof[(size_t)elems[i] * nd + a]]++] = (int)(i * nd + a);
+         }
+   }
+   LF_HIP(upload(h, h->prc_ptr, ptr.data(), ptr.size()));
+   LF_HIP(upload(h, h->prc_list, lst.data(), lst.size()));
+   LF_HIP(hipMalloc((void **)&h->Mhat, (size_t)nparam * nd * nd * sizeof(double2)));
+   // unit-eps mass matrices (face-transformed) of the parameter elements
+   LocalParams P = local_params(h, 0, LFASM_MASS, 1, h->Mhat);
+   P.nout = nparam;
+   P.elist = h->pelem;
+   P.eps_unit = 1;
+   LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   h->nparam = nparam;
+   return LFASM_OK;
+}
+
+int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int eliminate)
+{
+   if (!h || !h->nparam) { return h ? h->fail(LFASM_E_STATE, "variants_prepare first") : LFASM_E_STATE; }
+   if (nvar <= 0 || !deps) { return h->fail(LFASM_E_ARG, "no variants"); }
+   LF_HIP(hipSetDevice(h->device));
+   if (nvar != h->nvar)
+   {
+      dfree(h->var_vals);
+      LF_HIP(hipMalloc((void **)&h->var_vals, (size_t)nvar * h->nnz * sizeof(double2)));
+      h->nvar = nvar;
+   }
+   double2 *d_deps = nullptr;
+   LF_HIP(upload(h, d_deps, (const double2 *)deps, (size_t)nvar * h->nparam));
+   VariantParams V;
+   V.n = h->height;
+   V.nd = h->nd;
+   V.nvar = nvar;
+   V.nparam = h->nparam;
+   V.nnz = h->nnz;
+   V.Ip = h->Ip;
+   V.eflag = h->eflag;
+   V.prc_ptr = h->prc_ptr;
+   V.prc_list = h->prc_list;
+   V.pelem = h->pelem;
+   V.slot = h->slot;
+   V.is_ess = h->ess;
+   V.Mhat = h->Mhat;
+   V.deps = d_deps;
+   V.base = h->vals;
+   V.out = h->var_vals;
+   V.eliminate = eliminate ? 1 : 0;
+   V.maxrow = h->maxrow;
+   const size_t shmem = (size_t)4 * 2 * h->maxrow * sizeof(double2);
+   hipError_t e = hipFuncSetAttribute((const void *)k_variant_rows,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_variant_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem, h->stream, V);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess) { e = ev_end(h, T_BATCH); }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   (void)hipFree(d_deps);
+   LF_HIP(e);
+   return LFASM_OK;
+}
+
+int lfasm_variant_values(lfasm_handle *h, int v, void **dev_vals)
+{
+   if (!h || v < 0 || v >= h->nvar || !dev_vals) { return h ? h->fail(LFASM_E_ARG, "variant index") : LFASM_E_STATE; }
+   *dev_vals = (void *)(h->var_vals + (size_t)v * h->nnz);
+   return LFASM_OK;
+}
+
+int lfasm_download_variant(lfasm_handle *h, int v, double *re, double *im)
+{
+   if (!h || v < 0 || v >= h->nvar) { return h ? h->fail(LFASM_E_ARG, "variant index") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   std::vector<double2> t((size_t)h->nnz);
+   LF_HIP(hipMemcpyAsync(t.data(), h->var_vals + (size_t)v * h->nnz, t.size() * sizeof(double2),
+                         hipMemcpyDeviceToHost, h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   for (size_t k = 0; k < t.size(); ++k)
+   {
+      if (re) { re[k] = t[k].x; }
+      if (im) { im[k] = t[k].y; }
+   }
+   return LFASM_OK;
+}
+
+// ---- incident fields, RHS, sensitivity ---------------------------------------
+
+int lfasm_set_incident(lfasm_handle *h, const lfasm_incident *inc)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   if (!inc || inc->nrhs <= 0 || inc->nslab <= 0 || !inc->waves || !inc->region_slab ||
+       !inc->region_deps || !inc->region_src)
+   {
+      return h->fail(LFASM_E_ARG, "incomplete incident description");
+   }
+   for (int r = 0; r < h->nreg; ++r)
+   {
+      if (inc->region_slab[r] < 0 || inc->region_slab[r] >= inc->nslab)
+      {
+         return h->fail(LFASM_E_ARG, "region %d slab %d outside [0, %d)", r, inc->region_slab[r],
+                        inc->nslab);
+      }
+   }
+   LF_HIP(hipSetDevice(h->device));
+   const size_t nw = (size_t)h->nmesh * inc->nrhs * inc->nslab;
+   std::vector<SlabWaveDev> w(nw);
+   for (size_t i = 0; i < nw; ++i)
+   {
+      const double *x = inc->waves + 18 * i;
+      w[i].zA = x[0];
+      w[i].zB = x[1];
+      w[i].qA = make_double2(x[2], x[3]);
+      w[i].qB = make_double2(x[4], x[5]);
+      for (int c = 0; c < 3; ++c)
+      {
+         w[i].A[c] = make_double2(x[6 + 2 * c], x[7 + 2 * c]);
+         w[i].B[c] = make_double2(x[12 + 2 * c], x[13 + 2 * c]);
+      }
+   }
+   LF_HIP(upload(h, h->waves, w.data(), w.size()));
+   LF_HIP(upload(h, h->region_slab, inc->region_slab, (size_t)h->nreg));
+   LF_HIP(upload(h, h->region_src, inc->region_src, (size_t)h->nreg));
+   LF_HIP(upload(h, h->region_deps, (const double2 *)inc->region_deps, (size_t)h->nreg));
+   h->inc_nrhs = inc->nrhs;
+   h->inc_nslab = inc->nslab;
+   h->region_src_host.assign(inc->region_src, inc->region_src + h->nreg);
+   return LFASM_OK;
+}
+
+int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
+{
+   if (!h || !h->global_built || !h->inc_nrhs)
+   {
+      return h ? h->fail(LFASM_E_STATE, "build_global and set_incident first") : LFASM_E_STATE;
+   }
+   LF_HIP(hipSetDevice(h->device));
+   // source elements: those whose region is a source region
+   std::vector<int> attr((size_t)h->ne), pid((size_t)h->ne, 0), eps_off((size_t)h->nmesh);
+   LF_HIP(hipMemcpy(attr.data(), h->attr, attr.size() * sizeof(int), hipMemcpyDeviceToHost));
+   if (h->pid) { LF_HIP(hipMemcpy(pid.data(), h->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost)); }
+   LF_HIP(hipMemcpy(eps_off.data(), h->eps_off, eps_off.size() * sizeof(int), hipMemcpyDeviceToHost));
+   std::vector<int> src, index((size_t)h->ne, -1);
+   for (int e = 0; e < h->ne; ++e)
+   {
+      const int reg = eps_off[(size_t)pid[(size_t)e]] + attr[(size_t)e] - 1;
+      if (h->region_src_host[(size_t)reg]) { index[(size_t)e] = (int)src.size(); src.push_back(e); }
+   }
+   const int nrhs = h->inc_nrhs, nd = h->nd;
+   dfree(h->rhs);
+   LF_HIP(hipMalloc((void **)&h->rhs, (size_t)nrhs * h->height * sizeof(double2)));
+   int *d_src = nullptr, *d_index = nullptr;
+   double2 *bloc = nullptr;
+   hipError_t e = upload(h, d_index, index.data(), index.size());
+   if (e == hipSuccess && !src.empty()) { e = upload(h, d_src, src.data(), src.size()); }
+   if (e == hipSuccess && !src.empty())
+   {
+      e = hipMalloc((void **)&bloc, src.size() * nrhs * nd * sizeof(double2));
+   }
+   if (e == hipSuccess && !src.empty())
+   {
+      RhsParams R;
+      R.nd = nd;
+      R.p = h->p;
+      R.nrhs = nrhs;
+      R.nslab = h->inc_nslab;
+      R.nsrc = (int)src.size();
+      R.src_elem = d_src;
+      R.attr = h->attr;
+      R.pid = h->pid;
+      R.eps_off = h->eps_off;
+      R.J = h->J;
+      R.Ji = h->Ji;
+      R.detJ = h->detJ;
+      R.v0 = h->v0;
+      R.fo = h->fo;
+      R.rule = RuleDev{h->rm.nq, h->rm.ip, h->rm.w, h->rm.vs, h->rm.cs};
+      R.pml_tab = h->pml_tab;
+      R.region_slab = h->region_slab;
+      R.region_deps = h->region_deps;
+      R.eps_tab = h->eps_tab;
+      R.eps_elem = h->eps_elem;
+      R.waves = h->waves;
+      std::memcpy(R.TInv, h->TInv, sizeof(R.TInv));
+      R.bloc = bloc;
+      const size_t shmem = (size_t)h->rm.nq * nrhs * 3 * sizeof(double2);
+      e = hipFuncSetAttribute((const void *)k_rhs_local, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shmem);
+      if (e == hipSuccess)
+      {
+         hipLaunchKernelGGL(k_rhs_local, dim3((unsigned)src.size()), dim3(64), shmem, h->stream, R);
+         e = hipGetLastError();
+      }
+   }
+   if (e == hipSuccess)
+   {
+      if (src.empty())
+      {
+         hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, h->stream, h->rhs,
+                            (size_t)nrhs * h->height);
+      }
+      else
+      {
+         hipLaunchKernelGGL(k_rhs_rows, dim3((h->height + 255) / 256), dim3(256), 0, h->stream,
+                            h->height, nd, nrhs, h->rc_ptr, h->rc_list, d_index, h->eflag, h->ess,
+                            eliminate ? 1 : 0, bloc, h->rhs);
+      }
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   if (d_src) { (void)hipFree(d_src); }
+   if (d_index) { (void)hipFree(d_index); }
+   if (bloc) { (void)hipFree(bloc); }
+   LF_HIP(e);
+   return LFASM_OK;
+}
+
+int lfasm_download_rhs(lfasm_handle *h, double *re, double *im)
+{
+   if (!h || !h->rhs) { return h ? h->fail(LFASM_E_STATE, "assemble_rhs first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   std::vector<double2> t((size_t)h->inc_nrhs * h->height);
+   LF_HIP(hipMemcpyAsync(t.data(), h->rhs, t.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   for (size_t k = 0; k < t.size(); ++k)
+   {
+      if (re) { re[k] = t[k].x; }
+      if (im) { im[k] = t[k].y; }
+   }
+   return LFASM_OK;
+}
+
+int lfasm_eps_sensitivity(lfasm_handle *h, int nparam, const int *elems, const double *lam,
+                          const double *u, int rhs_col, double *s_out)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   if (nparam <= 0 || !elems || !lam || !u || !s_out) { return h->fail(LFASM_E_ARG, "null argument"); }
+   if (rhs_col >= 0 && rhs_col >= h->inc_nrhs) { return h->fail(LFASM_E_ARG, "rhs_col without incident field"); }
+   for (int i = 0; i < nparam; ++i)
+   {
+      if (elems[i] < 0 || elems[i] >= h->ne) { return h->fail(LFASM_E_ARG, "element %d out of range", elems[i]); }
+   }
+   LF_HIP(hipSetDevice(h->device));
+   int *d_el = nullptr;
+   double2 *d_lam = nullptr, *d_u = nullptr, *d_s = nullptr;
+   hipError_t e = upload(h, d_el, elems, (size_t)nparam);
+   if (e == hipSuccess) { e = upload(h, d_lam, (const double2 *)lam, (size_t)h->height); }
+   if (e == hipSuccess) { e = upload(h, d_u, (const double2 *)u, (size_t)h->height); }
+   if (e == hipSuccess) { e = hipMalloc((void **)&d_s, (size_t)nparam * sizeof(double2)); }
+   if (e == hipSuccess)
+   {
+      SensParams S;
+      S.nd = h->nd;
+      S.p = h->p;
+      S.nparam = nparam;
+      S.nrhs = h->inc_nrhs;
+      S.nslab = h->inc_nslab;
+      S.rhs_col = rhs_col;
+      S.pelem = d_el;
+      S.dof = h->dof;
+      S.attr = h->attr;
+      S.pid = h->pid;
+      S.eps_off = h->eps_off;
+      S.fo = h->fo;
+      S.eflag = h->eflag;
+      S.J = h->J;
+      S.Ji = h->Ji;
+      S.detJ = h->detJ;
+      S.v0 = h->v0;
+      S.rm = RuleDev{h->rm.nq, h->rm.ip, h->rm.w, h->rm.vs, h->rm.cs};
+      S.pml_tab = h->pml_tab;
+      S.region_slab = h->region_slab;
+      S.region_src = h->region_src;
+      S.waves = h->waves;
+      std::memcpy(S.TInv, h->TInv, sizeof(S.TInv));
+      S.lam = d_lam;
+      S.u = d_u;
+      S.s = d_s;
+      e = ev_begin(h, T_ADJ);
+      if (e == hipSuccess)
+      {
+         hipLaunchKernelGGL(k_eps_sens, dim3(nparam), dim3(64), 2 * h->nd * sizeof(double2),
+                            h->stream, S);
+         e = hipGetLastError();
+      }
+      if (e == hipSuccess) { e = ev_end(h, T_ADJ); }
+   }
+   if (e == hipSuccess)
+   {
+      e = hipMemcpyAsync(s_out, d_s, (size_t)nparam * sizeof(double2), hipMemcpyDeviceToHost, h->stream);
+   }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   if (d_el) { (void)hipFree(d_el); }
+   if (d_lam) { (void)hipFree(d_lam); }
+   if (d_u) { (void)hipFree(d_u); }
+   if (d_s) { (void)hipFree(d_s); }
+   LF_HIP(e);
    return LFASM_OK;
 }
 

@@ -1,0 +1,389 @@
+// Batched and adjoint device code of the gfx950 Nedelec assembly.  None of
+// these have a reference counterpart; they build on the reference's
+// assembly (solver/asm_gpu.inc) and RHS definition
+// (solver/problem_common.inc:218-266):
+//
+// * eps variants: A(eps_v) = A(eps_base) + sum_{e in params} deps_{v,e} Mhat_e,
+//   where Mhat_e is the face-transformed mass element matrix at unit eps
+//   (the mass term is linear in eps); one wavefront per CSR row writes the
+//   row of every variant, contributions in ascending element order;
+// * batched RHS: b_k = int k0^2 deps_region E_inc(z) . v_k over scattered-
+//   field source (frustum) regions, for many incident fields at once;
+// * eps sensitivity: s_e = lam_e^T (dA_e/deps_e u_e - db_e/deps_e)
+//   = -k0^2 int L(x) . (Lam U(x) + f_e E_inc(z)) with L, U the adjoint and
+//   forward fields of the element.
+#pragma once
+
+#include "lfasm_device.hpp"
+
+namespace lfasm {
+
+// ---- eps variants -------------------------------------------------------------
+
+struct VariantParams {
+   int n, nd, nvar, nparam;
+   int64_t nnz;
+   const int *Ip;
+   const unsigned char *eflag;   // per (e, b): bit0 sign < 0, bit1 essential
+   const int *prc_ptr, *prc_list;  // per row: param contributors pi*nd + a, ascending element
+   const int *pelem;             // param index -> element
+   const uint16_t *slot;         // element slot map
+   const unsigned char *is_ess;
+   const double2 *Mhat;          // nparam x nd x nd
+   const double2 *deps;          // nvar x nparam
+   const double2 *base;          // nnz
+   double2 *out;                 // nvar x nnz
+   int eliminate, maxrow;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
+{
+   extern __shared__ double2 vbuf[];
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + wave;
+   if (r >= V.n) { return; }
+   double2 *row0 = vbuf + (size_t)wave * 2 * V.maxrow;
+   double2 *buf = row0 + V.maxrow;
+   const int base = V.Ip[r];
+   const int len = V.Ip[r + 1] - base;
+   const int c0 = V.prc_ptr[r], c1 = V.prc_ptr[r + 1];
+   const bool row_fixed = c0 == c1 || (V.eliminate && V.is_ess[r]);
+   if (row_fixed)
+   {
+      for (int v = 0; v < V.nvar; ++v)
+      {
+         for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = V.base[base + s]; }
+      }
+      return;
+   }
+   for (int s = lane; s < len; s += 64) { row0[s] = V.base[base + s]; }
+   const int nd = V.nd;
+   const unsigned char ess_bit = V.eliminate ? 2 : 0;
+   for (int v = 0; v < V.nvar; ++v)
+   {
+      wave_sync_lds();
+      for (int s = lane; s < len; s += 64) { buf[s] = row0[s]; }
+      wave_sync_lds();
+      for (int c = c0; c < c1; ++c)
+      {
+         const int pa = V.prc_list[c];
+         const int pi = pa / nd, a = pa - pi * nd;
+         const int e = V.pelem[pi];
+         const int ea = e * nd + a;
+         const double2 d = V.deps[(size_t)v * V.nparam + pi];
+         const unsigned char fa = V.eflag[ea];
+         for (int b = lane; b < nd; b += 64)
+         {
+            const unsigned char fb = V.eflag[(size_t)e * nd + b];
+            if (fb & ess_bit) { continue; }
+            const int s = V.slot[(size_t)ea * nd + b];
+            const double2 m = V.Mhat[((size_t)pi * nd + a) * nd + b];
+            double2 t = cmul(d, m);
+            if ((fa ^ fb) & 1) { t.x = -t.x; t.y = -t.y; }
+            double2 x = buf[s];
+            x.x += t.x;
+            x.y += t.y;
+            buf[s] = x;
+         }
+         wave_sync_lds();
+      }
+      for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = buf[s]; }
+   }
+}
+
+// ---- incident field ---------------------------------------------------------------
+
+// One slab wave (the reference's SlabWave, solver/problem_common.inc:11-16):
+// E(z) = A exp(i qA (z - zA)) + B exp(i qB (z - zB)).
+struct SlabWaveDev {
+   double zA, zB;
+   double2 qA, qB;
+   double2 A[3], B[3];
+};
+
+__device__ __forceinline__ double2 cexpi(double2 q, double dz)
+{
+   // exp(i q dz) = exp(-Im(q) dz) (cos(Re(q) dz) + i sin(Re(q) dz))
+   const double mag = exp(-q.y * dz);
+   double sn, cs;
+   sincos(q.x * dz, &sn, &cs);
+   return make_double2(mag * cs, mag * sn);
+}
+
+__device__ __forceinline__ void incident_at(const SlabWaveDev &w, double z, double2 E[3])
+{
+   const double2 up = cexpi(w.qA, z - w.zA);
+   const double2 dn = cexpi(w.qB, z - w.zB);
+#pragma unroll
+   for (int i = 0; i < 3; ++i)
+   {
+      const double2 a = cmul(w.A[i], up), b = cmul(w.B[i], dn);
+      E[i] = make_double2(a.x + b.x, a.y + b.y);
+   }
+}
+
+struct RhsParams {
+   int nd, p, nrhs, nslab;
+   int nsrc;                      // source elements (list below)
+   const int *src_elem;           // source element ids
+   const int *attr, *pid, *eps_off;
+   const double *J, *Ji, *detJ, *v0;
+   const int *fo;
+   RuleDev rule;                  // quadrature rule of the linear form
+   const PmlDev *pml_tab;         // k0 per problem
+   const int *region_slab;        // per region (concatenated per problem)
+   const double2 *region_deps;    // per region: eps - eps_bg
+   const double2 *eps_tab;        // region eps (concatenated per problem)
+   const double2 *eps_elem;       // optional per-element eps
+   const SlabWaveDev *waves;      // [pid][rhs][slab]
+   double TInv[24];
+   double2 *bloc;                 // nsrc x nrhs x nd
+};
+
+// One wavefront per source element: the fields at the rule points are staged
+// in LDS, each lane then integrates one basis function against every
+// incident field, and the dual face transform is applied to the vector.
+__global__ void __launch_bounds__(64) k_rhs_local(RhsParams R)
+{
+   extern __shared__ double2 efield[];        // nq x nrhs x 3
+   const int fi = blockIdx.x;
+   const int lane = threadIdx.x;
+   const int e = R.src_elem[fi];
+   const int pid = R.pid ? R.pid[e] : 0;
+   const int reg = R.eps_off[pid] + R.attr[e] - 1;
+   const int slab = R.region_slab[reg];
+   // source strength eps_e - eps_bg (eps_bg = region eps - region deps);
+   // without per-element eps this is the region's eps - eps_bg itself
+   double2 de = R.region_deps[reg];
+   if (R.eps_elem)
+   {
+      const double2 ee = R.eps_elem[e], er = R.eps_tab[reg];
+      de = make_double2(ee.x - (er.x - de.x), ee.y - (er.y - de.y));
+   }
+   const double k2 = R.pml_tab[pid].k0 * R.pml_tab[pid].k0;
+   const double det = R.detJ[e];
+   const double *J = R.J + (size_t)e * 9;
+   const double *Ji = R.Ji + (size_t)e * 9;
+   const double v0z = R.v0[(size_t)e * 3 + 2];
+   const int nq = R.rule.nq, nrhs = R.nrhs;
+   // f(z_q) = w det k0^2 deps E_inc(z_q) for every (q, rhs)
+   for (int it = lane; it < nq * nrhs; it += 64)
+   {
+      const int q = it / nrhs, r = it - q * nrhs;
+      const double *xi = R.rule.ip + (size_t)q * 3;
+      const double z = v0z + J[6] * xi[0] + J[7] * xi[1] + J[8] * xi[2];
+      double2 E[3];
+      incident_at(R.waves[((size_t)pid * nrhs + r) * R.nslab + slab], z, E);
+      const double2 f = make_double2(R.rule.w[q] * det * k2 * de.x, R.rule.w[q] * det * k2 * de.y);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) { efield[(size_t)it * 3 + i] = cmul(f, E[i]); }
+   }
+   __syncthreads();
+   const int nd = R.nd, p = R.p;
+   for (int k0 = 0; k0 < nd; k0 += 64)
+   {
+      const int k = k0 + lane;
+      const bool valid = k < nd;
+      for (int r = 0; r < nrhs; ++r)
+      {
+         double2 acc = make_double2(0.0, 0.0);
+         if (valid)
+         {
+            for (int q = 0; q < nq; ++q)
+            {
+               const double *vs = R.rule.vs + ((size_t)q * nd + k) * 3;
+               const double a0 = vs[0], a1 = vs[1], a2 = vs[2];
+               const double v[3] = {a0 * Ji[0] + a1 * Ji[3] + a2 * Ji[6],
+                                    a0 * Ji[1] + a1 * Ji[4] + a2 * Ji[7],
+                                    a0 * Ji[2] + a1 * Ji[5] + a2 * Ji[8]};
+               const double2 *f = efield + ((size_t)q * nrhs + r) * 3;
+#pragma unroll
+               for (int i = 0; i < 3; ++i)
+               {
+                  acc.x += v[i] * f[i].x;
+                  acc.y += v[i] * f[i].y;
+               }
+            }
+         }
+         // dual face transform of the element vector (pairs in lanes k, k^1)
+         if (p >= 2)
+         {
+            const int foff = 6 * p, ntd = p * (p - 1);
+            double cs = 1.0, cp = 0.0;
+            if (valid && k >= foff && k < foff + 4 * ntd)
+            {
+               const double *Ti = R.TInv + 4 * R.fo[(size_t)e * 4 + (k - foff) / ntd];
+               const bool even = ((k - foff) & 1) == 0;
+               cs = even ? Ti[0] : Ti[3];
+               cp = even ? Ti[1] : Ti[2];
+            }
+            const double yx = __shfl_xor(acc.x, 1), yy = __shfl_xor(acc.y, 1);
+            acc = make_double2(cs * acc.x + cp * yx, cs * acc.y + cp * yy);
+         }
+         if (valid) { R.bloc[((size_t)fi * nrhs + r) * nd + k] = acc; }
+      }
+   }
+}
+
+// b[rhs][row] = sum over the row's contributors (ascending element) of
+// sign * bloc; essential rows are zero with `eliminate`.
+__global__ void k_rhs_rows(int n, int nd, int nrhs, const int *__restrict__ rc_ptr,
+                           const int *__restrict__ rc_list, const int *__restrict__ src_index,
+                           const unsigned char *__restrict__ eflag,
+                           const unsigned char *__restrict__ is_ess, int eliminate,
+                           const double2 *__restrict__ bloc, double2 *__restrict__ b)
+{
+   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= n) { return; }
+   const bool zero = eliminate && is_ess[r];
+   for (int q = 0; q < nrhs; ++q)
+   {
+      double2 acc = make_double2(0.0, 0.0);
+      if (!zero)
+      {
+         for (int c = rc_ptr[r]; c < rc_ptr[r + 1]; ++c)
+         {
+            const int ea = rc_list[c];
+            const int e = ea / nd, a = ea - e * nd;
+            const int fi = src_index[e];
+            if (fi < 0) { continue; }
+            const double2 v = bloc[((size_t)fi * nrhs + q) * nd + a];
+            if (eflag[ea] & 1) { acc.x -= v.x; acc.y -= v.y; }
+            else { acc.x += v.x; acc.y += v.y; }
+         }
+      }
+      b[(size_t)q * n + r] = acc;
+   }
+}
+
+// ---- eps sensitivity (adjoint) --------------------------------------------------
+
+struct SensParams {
+   int nd, p, nparam, nrhs, nslab, rhs_col;
+   const int *pelem;
+   const int *dof, *attr, *pid, *eps_off, *fo;
+   const unsigned char *eflag;
+   const double *J, *Ji, *detJ, *v0;
+   RuleDev rm;                    // mass rule
+   const PmlDev *pml_tab;
+   const int *region_slab;        // may be NULL when rhs_col < 0
+   const int *region_src;         // 1 for scattered-field source regions
+   const SlabWaveDev *waves;
+   double TInv[24];
+   const double2 *lam, *u;        // global adjoint / forward vectors
+   double2 *s;                    // nparam
+};
+
+// One wavefront per parameter element: gather the local vectors (sign and
+// face-pair transform), then lane q evaluates L(x_q), U(x_q) and the
+// integrand at point q; a wave reduction sums the points.
+__global__ void __launch_bounds__(64) k_eps_sens(SensParams S)
+{
+   extern __shared__ double2 loc[];            // 2 x nd
+   const int pi = blockIdx.x;
+   const int lane = threadIdx.x;
+   const int e = S.pelem[pi];
+   const int nd = S.nd, p = S.p;
+   double2 *lam = loc, *uu = loc + nd;
+   for (int k0 = 0; k0 < nd; k0 += 64)
+   {
+      const int k = k0 + lane;
+      const bool valid = k < nd;
+      double2 l = make_double2(0.0, 0.0), u = make_double2(0.0, 0.0);
+      if (valid)
+      {
+         const int d = S.dof[(size_t)e * nd + k];
+         const double sg = (S.eflag[(size_t)e * nd + k] & 1) ? -1.0 : 1.0;
+         l = S.lam[d];
+         u = S.u[d];
+         l.x *= sg; l.y *= sg;
+         u.x *= sg; u.y *= sg;
+      }
+      if (p >= 2)
+      {
+         // local pair = M g, M = [[t0, t2], [t1, t3]] (TInv column-major)
+         const int foff = 6 * p, ntd = p * (p - 1);
+         double cs = 1.0, cp = 0.0;
+         if (valid && k >= foff && k < foff + 4 * ntd)
+         {
+            const double *Ti = S.TInv + 4 * S.fo[(size_t)e * 4 + (k - foff) / ntd];
+            const bool even = ((k - foff) & 1) == 0;
+            cs = even ? Ti[0] : Ti[3];
+            cp = even ? Ti[2] : Ti[1];
+         }
+         const double lx = __shfl_xor(l.x, 1), ly = __shfl_xor(l.y, 1);
+         const double ux = __shfl_xor(u.x, 1), uy = __shfl_xor(u.y, 1);
+         l = make_double2(cs * l.x + cp * lx, cs * l.y + cp * ly);
+         u = make_double2(cs * u.x + cp * ux, cs * u.y + cp * uy);
+      }
+      if (valid)
+      {
+         lam[k] = l;
+         uu[k] = u;
+      }
+   }
+   __syncthreads();
+   const int pid = S.pid ? S.pid[e] : 0;
+   const PmlDev pm = S.pml_tab[pid];
+   const int reg = S.eps_off[pid] + S.attr[e] - 1;
+   const bool src = S.rhs_col >= 0 && S.region_src && S.region_src[reg];
+   const double det = S.detJ[e];
+   const double *J = S.J + (size_t)e * 9;
+   const double *Ji = S.Ji + (size_t)e * 9;
+   const double v0z = S.v0[(size_t)e * 3 + 2];
+   double2 acc = make_double2(0.0, 0.0);
+   for (int q = lane; q < S.rm.nq; q += 64)
+   {
+      double2 L[3] = {}, U[3] = {};
+      for (int k = 0; k < nd; ++k)
+      {
+         const double *vs = S.rm.vs + ((size_t)q * nd + k) * 3;
+         const double a0 = vs[0], a1 = vs[1], a2 = vs[2];
+         const double v[3] = {a0 * Ji[0] + a1 * Ji[3] + a2 * Ji[6],
+                              a0 * Ji[1] + a1 * Ji[4] + a2 * Ji[7],
+                              a0 * Ji[2] + a1 * Ji[5] + a2 * Ji[8]};
+         const double2 lk = lam[k], uk = uu[k];
+#pragma unroll
+         for (int i = 0; i < 3; ++i)
+         {
+            L[i].x += v[i] * lk.x; L[i].y += v[i] * lk.y;
+            U[i].x += v[i] * uk.x; U[i].y += v[i] * uk.y;
+         }
+      }
+      const double *xi = S.rm.ip + (size_t)q * 3;
+      const double z = v0z + J[6] * xi[0] + J[7] * xi[1] + J[8] * xi[2];
+      const double2 s = pml_s(pm, z);
+      const double2 lamd[3] = {s, s, cinv(s)};   // Lam = diag(s, s, 1/s)
+      double2 E[3] = {};
+      if (src)
+      {
+         incident_at(S.waves[((size_t)pid * S.nrhs + S.rhs_col) * S.nslab + S.region_slab[reg]], z, E);
+      }
+      const double w = S.rm.w[q] * det;
+      double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+      {
+         const double2 lu = cmul(lamd[i], U[i]);
+         const double2 f = make_double2(lu.x + E[i].x, lu.y + E[i].y);
+         const double2 g = cmul(L[i], f);
+         t.x += g.x;
+         t.y += g.y;
+      }
+      acc.x += w * t.x;
+      acc.y += w * t.y;
+   }
+   // wave reduction over the points
+   for (int off = 32; off > 0; off >>= 1)
+   {
+      acc.x += __shfl_xor(acc.x, off);
+      acc.y += __shfl_xor(acc.y, off);
+   }
+   if (lane == 0)
+   {
+      const double k2 = pm.k0 * pm.k0;
+      S.s[pi] = make_double2(-k2 * acc.x, -k2 * acc.y);
+   }
+}
+
+}  // namespace lfasm

@@ -39,12 +39,17 @@ struct LocalParams {
    const double2 *eps_elem;
    const int *fo;
    RuleDev rc, rm;
-   PmlDev pml;
+   const PmlDev *pml_tab;   // per problem (one entry for a single mesh)
+   const int *pid;          // element -> problem id (NULL: problem 0)
+   const int *eps_off;      // problem -> offset of its region table in eps_tab
    double TInv[24];
    int use_cc;       // curl-curl stream on the cc rule
    int mflags;       // bit0 mass, bit1 cross0, bit2 cross1, bit3 klk
    int apply_te;
    double2 *Ae;
+   int nout;             // number of element matrices written (launch size)
+   const int *elist;     // output i -> element elist[i] (NULL: element i)
+   int eps_unit;         // 1: eps = 1 in the mass term (unit-eps mass matrices)
 };
 
 // ---- complex helpers ------------------------------------------------------
@@ -79,6 +84,7 @@ struct ElemGeo {
    double Ji[9];    // row-major J^{-1}
    double J6, J7, J8, v0z, detJ;
    double2 eps;
+   PmlDev pm;       // PML / Bloch parameters of the element's problem
 };
 
 __device__ __forceinline__ void load_geo(const LocalParams &P, int e, ElemGeo &g)
@@ -96,7 +102,10 @@ __device__ __forceinline__ void load_geo(const LocalParams &P, int e, ElemGeo &g
    g.J8 = P.J[(size_t)e * 9 + 8];
    g.v0z = P.v0[(size_t)e * 3 + 2];
    g.detJ = det;
-   g.eps = P.eps_elem ? P.eps_elem[e] : P.eps_tab[P.attr[e] - 1];
+   const int pid = P.pid ? P.pid[e] : 0;
+   g.pm = P.pml_tab[pid];
+   g.eps = P.eps_unit ? make_double2(1.0, 0.0)
+           : P.eps_elem ? P.eps_elem[e] : P.eps_tab[P.eps_off[pid] + P.attr[e] - 1];
 }
 
 // Per quadrature point q (cc rule first, then mass rule): w*Lam^-1 diagonal
@@ -108,7 +117,7 @@ __device__ __forceinline__ void q_coef(const LocalParams &P, const ElemGeo &g, i
    const int qq = (q < P.rc.nq) ? q : q - P.rc.nq;
    const double *xi = R.ip + (size_t)qq * 3;
    const double z = g.v0z + g.J6 * xi[0] + g.J7 * xi[1] + g.J8 * xi[2];
-   const double2 s = pml_s(P.pml, z);
+   const double2 s = pml_s(g.pm, z);
    const double2 si = cinv(s);
    const double w = R.w[qq] * g.detJ;
    qc[0] = w * si.x;
@@ -154,14 +163,15 @@ struct MassCoef {
    double m0r, m0i, m1r, m1i, m2r, m2i, kxy, c0x, c0y, c1x, c1y;
 };
 
-__device__ __forceinline__ MassCoef mass_coef(const LocalParams &P, double2 eps)
+__device__ __forceinline__ MassCoef mass_coef(const LocalParams &P, const PmlDev &pm,
+                                              double2 eps)
 {
    const double fm = (P.mflags & 1) ? 1.0 : 0.0;
    const double f0 = (P.mflags & 2) ? 1.0 : 0.0;
    const double f1 = (P.mflags & 4) ? 1.0 : 0.0;
    const double fk = (P.mflags & 8) ? 1.0 : 0.0;
-   const double kx = P.pml.kx, ky = P.pml.ky;
-   const double k2 = P.pml.k0 * P.pml.k0;
+   const double kx = pm.kx, ky = pm.ky;
+   const double k2 = pm.k0 * pm.k0;
    const double mer = -k2 * eps.x * fm, mei = -k2 * eps.y * fm;
    MassCoef c;
    c.m0r = mer + fk * ky * ky;
@@ -204,7 +214,7 @@ __device__ __forceinline__ void elem_setup(const LocalParams &P, int e, int nqt,
    ElemGeo g;
    load_geo(P, e, g);
    for (int q = tid; q < nqt; q += nthr) { q_coef(P, g, q, qcoef + q * 4); }
-   if (tid == 0) { store_elem_const(g, mass_coef(P, g.eps), ec); }
+   if (tid == 0) { store_elem_const(g, mass_coef(P, g.pm, g.eps), ec); }
 }
 
 // Contravariant Piola c = (J/detJ) chat, covariant v = J^{-T} vhat.
@@ -355,7 +365,8 @@ typedef double dbl2v __attribute__((ext_vector_type(2)));
 // (row tiles wsub*RT .. wsub*RT+RT-1, all NT column tiles).
 template <int NT, int RT>
 __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[RT][NT][2],
-                                              int e, bool live, int wsub, int g4, int m16)
+                                              int e, int io, bool live, int wsub, int g4,
+                                              int m16)
 {
    // dual face-DOF transform TInv(fo)^T A TInv: columns, then rows
    // (solver/asm_gpu.inc:221-251)
@@ -417,7 +428,7 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
    }
 
    if (!live) { return; }
-   double2 *Ae = P.Ae + (size_t)e * nd * nd;
+   double2 *Ae = P.Ae + (size_t)io * nd * nd;
 #pragma unroll
    for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -447,9 +458,10 @@ k_local_mfma(LocalParams P, int nqt)
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
    const int slot = wave / WPE, wsub = wave - slot * WPE;
-   const int e_raw = blockIdx.x * EPB + slot;
-   const bool live = e_raw < P.ne;
-   const int e = live ? e_raw : P.ne - 1;
+   const int i_raw = blockIdx.x * EPB + slot;
+   const bool live = i_raw < P.nout;
+   const int io = live ? i_raw : P.nout - 1;
+   const int e = P.elist ? P.elist[io] : io;
 
    extern __shared__ double smem[];
    const int slot_doubles = EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP;
@@ -505,7 +517,7 @@ k_local_mfma(LocalParams P, int nqt)
       __syncthreads();
    }
 
-   mfma_epilogue<NT, RT>(P, acc, e, live, wsub, g4, m16);
+   mfma_epilogue<NT, RT>(P, acc, e, io, live, wsub, g4, m16);
 }
 
 // Pipelined variant: one row tile per wavefront (NT wavefronts per
@@ -523,9 +535,10 @@ k_local_mfma_pipe(LocalParams P, int nqt)
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
    const int slot = wave / NT, wsub = wave - slot * NT;
-   const int e_raw = blockIdx.x * EPB + slot;
-   const bool live = e_raw < P.ne;
-   const int e = live ? e_raw : P.ne - 1;
+   const int i_raw = blockIdx.x * EPB + slot;
+   const bool live = i_raw < P.nout;
+   const int io = live ? i_raw : P.nout - 1;
+   const int e = P.elist ? P.elist[io] : io;
 
    extern __shared__ double smem[];
    const int slot_doubles = EC_SIZE + nqt * 4 + 2 * STG;
@@ -585,7 +598,7 @@ k_local_mfma_pipe(LocalParams P, int nqt)
       }
       __syncthreads();
    }
-   mfma_epilogue<NT, 1>(P, acc, e, live, wsub, g4, m16);
+   mfma_epilogue<NT, 1>(P, acc, e, io, live, wsub, g4, m16);
 }
 
 // ---- generic VALU local-matrix kernel (any nd) -----------------------------
@@ -597,7 +610,8 @@ template <int NACC, int NTHR>
 __global__ void __launch_bounds__(NTHR)
 k_local_generic(LocalParams P, int nqt, int NP)
 {
-   const int e = blockIdx.x;
+   const int io = blockIdx.x;
+   const int e = P.elist ? P.elist[io] : io;
    const int tid = threadIdx.x;
    extern __shared__ double smem[];
    double *ec = smem;
@@ -640,7 +654,7 @@ k_local_generic(LocalParams P, int nqt, int NP)
       }
       __syncthreads();
    }
-   double2 *Ae = P.Ae + (size_t)e * n2;
+   double2 *Ae = P.Ae + (size_t)io * n2;
 #pragma unroll
    for (int i = 0; i < NACC; ++i)
    {
@@ -651,11 +665,12 @@ k_local_generic(LocalParams P, int nqt, int NP)
 
 // Dual face-DOF transform in place on Ae (solver/asm_gpu.inc:221-251):
 // one block per element, columns then rows.
-__global__ void k_te(int nd, int p, const int *__restrict__ fo, const double *TInvG,
-                     double2 *__restrict__ Ae)
+__global__ void k_te(int nd, int p, const int *__restrict__ fo, const int *__restrict__ elist,
+                     const double *TInvG, double2 *__restrict__ Ae)
 {
-   const int e = blockIdx.x;
-   double2 *A = Ae + (size_t)e * nd * nd;
+   const int io = blockIdx.x;
+   const int e = elist ? elist[io] : io;
+   double2 *A = Ae + (size_t)io * nd * nd;
    const int foff = 6 * p;
    const int ntd = p * (p - 1);
    for (int axis = 0; axis < 2; ++axis)

@@ -51,6 +51,12 @@ class _Pml(ct.Structure):
                 ("z_min", "z_max", "thick", "order", "smax_bot", "smax_top", "k0", "kx", "ky")]
 
 
+class _Incident(ct.Structure):
+    _fields_ = [("nrhs", ct.c_int), ("nslab", ct.c_int), ("waves", ct.c_void_p),
+                ("region_slab", ct.c_void_p), ("region_deps", ct.c_void_p),
+                ("region_src", ct.c_void_p)]
+
+
 class _Mesh(ct.Structure):
     _fields_ = [("ne", ct.c_int), ("nd", ct.c_int), ("p", ct.c_int), ("height", ct.c_int),
                 ("dof", ct.c_void_p), ("sign", ct.c_void_p), ("fo", ct.c_void_p),
@@ -101,6 +107,16 @@ def lib():
         "lfasm_csr_symbolic": ([i, i, i, vp, vp, ct.POINTER(vp)], i64),
         "lfasm_free": ([vp], None),
         "lfasm_measure_peaks": ([i, vp], i),
+        "lfasm_init_batch": ([vp, vp, i], i),
+        "lfasm_batch_offsets": ([vp, vp, vp], i),
+        "lfasm_variants_prepare": ([vp, i, vp], i),
+        "lfasm_variants_assemble": ([vp, i, vp, i], i),
+        "lfasm_variant_values": ([vp, i, ct.POINTER(vp)], i),
+        "lfasm_download_variant": ([vp, i, vp, vp], i),
+        "lfasm_set_incident": ([vp, vp], i),
+        "lfasm_assemble_rhs": ([vp, i], i),
+        "lfasm_download_rhs": ([vp, vp, vp], i),
+        "lfasm_eps_sensitivity": ([vp, i, vp, vp, vp, i, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -178,9 +194,7 @@ class AsmGpuGlobal:
             raise AsmGpuError(rc, self.last_error())
 
     # -- AsmGpu::Init
-    def init(self, ad, problem, eps_elem=None):
-        keep = []
-
+    def _mesh_struct(self, ad, problem, eps_elem, keep):
         def k(a, dt):
             if a is None:
                 return None
@@ -192,20 +206,45 @@ class AsmGpuGlobal:
             return _Rule(r.nq, _p(k(r.ip, np.float64)), _p(k(r.w, np.float64)),
                          _p(k(r.vshape, np.float64)), _p(k(r.curl, np.float64)))
 
-        pml = problem.pml_params()
         ee = None
         if eps_elem is not None:
             ee = k(np.asarray(eps_elem, dtype=np.complex128).view(np.float64), np.float64)
-        m = _Mesh(ad.ne, ad.nd, ad.p, ad.height,
-                  _p(k(ad.dof, np.int32)), _p(k(ad.sign, np.float64)),
-                  _p(k(ad.fo, np.int32)) if ad.p >= 2 else None,
-                  _p(k(ad.attr, np.int32)), _p(k(ad.J, np.float64)), _p(k(ad.Jinv, np.float64)),
-                  _p(k(ad.detJ, np.float64)), _p(k(ad.v0, np.float64)),
-                  rule(ad.rule_cc), rule(ad.rule_mass), len(problem.regions),
-                  _p(k(problem.eps_tab(), np.float64)), _p(ee), _Pml(*pml),
-                  _p(k(ND_TINV, np.float64)))
+        return _Mesh(ad.ne, ad.nd, ad.p, ad.height,
+                     _p(k(ad.dof, np.int32)), _p(k(ad.sign, np.float64)),
+                     _p(k(ad.fo, np.int32)) if ad.p >= 2 else None,
+                     _p(k(ad.attr, np.int32)), _p(k(ad.J, np.float64)),
+                     _p(k(ad.Jinv, np.float64)), _p(k(ad.detJ, np.float64)),
+                     _p(k(ad.v0, np.float64)), rule(ad.rule_cc), rule(ad.rule_mass),
+                     len(problem.regions), _p(k(problem.eps_tab(), np.float64)), _p(ee),
+                     _Pml(*problem.pml_params()), _p(k(ND_TINV, np.float64)))
+
+    def init(self, ad, problem, eps_elem=None):
+        keep = []
+        m = self._mesh_struct(ad, problem, eps_elem, keep)
         self._check(lib().lfasm_init(self._h, ct.byref(m)))
         self.ne, self.nd, self.n = ad.ne, ad.nd, ad.height
+        self.nmesh = 1
+
+    def init_batch(self, items):
+        """items: sequence of (AsmData, Problem[, eps_elem]); the meshes are
+        assembled as one block-diagonal system."""
+        keep = []
+        structs = [self._mesh_struct(it[0], it[1], it[2] if len(it) > 2 else None, keep)
+                   for it in items]
+        arr = (_Mesh * len(structs))(*structs)
+        self._check(lib().lfasm_init_batch(self._h, arr, len(structs)))
+        self.ne = sum(it[0].ne for it in items)
+        self.nd = items[0][0].nd
+        self.n = sum(it[0].height for it in items)
+        self.nmesh = len(items)
+
+    def batch_offsets(self):
+        eo = np.zeros(self.nmesh + 1, np.int64)
+        ro = np.zeros(self.nmesh + 1, np.int64)
+        rc = lib().lfasm_batch_offsets(self._h, _p(eo), _p(ro))
+        if rc < 0:
+            self._check(rc)
+        return eo, ro
 
     def set_eps_elem(self, eps_elem):
         if eps_elem is None:
@@ -259,6 +298,54 @@ class AsmGpuGlobal:
         rel = ct.c_double()
         self._check(lib().lfasm_residual(self._h, _p(X2n), _p(B2n), ct.byref(rel)))
         return rel.value
+
+    # -- eps variants (delta assembly)
+    def variants_prepare(self, elems):
+        el = np.ascontiguousarray(np.asarray(elems, dtype=np.int32))
+        self._check(lib().lfasm_variants_prepare(self._h, len(el), _p(el)))
+        self.nparam = len(el)
+
+    def variants_assemble(self, deps, eliminate=True):
+        d = np.ascontiguousarray(np.asarray(deps, dtype=np.complex128))
+        assert d.ndim == 2 and d.shape[1] == self.nparam
+        self._check(lib().lfasm_variants_assemble(self._h, d.shape[0], _p(d.view(np.float64)),
+                                                  1 if eliminate else 0))
+        self.nvar = d.shape[0]
+
+    def download_variant(self, v):
+        re = np.empty(self.nnz)
+        im = np.empty(self.nnz)
+        self._check(lib().lfasm_download_variant(self._h, v, _p(re), _p(im)))
+        return re, im
+
+    # -- incident fields, RHS, sensitivity
+    def set_incident(self, inc):
+        """inc: lithofem_amd.incident.IncidentSet (or any object with
+        nrhs, nslab, waves (…, 18), region_slab, region_deps, region_src)."""
+        w = np.ascontiguousarray(inc.waves, dtype=np.float64)
+        rs = np.ascontiguousarray(inc.region_slab, dtype=np.int32)
+        rd = np.ascontiguousarray(np.asarray(inc.region_deps, dtype=np.complex128).view(np.float64))
+        rsrc = np.ascontiguousarray(inc.region_src, dtype=np.int32)
+        st = _Incident(inc.nrhs, inc.nslab, _p(w), _p(rs), _p(rd), _p(rsrc))
+        self._check(lib().lfasm_set_incident(self._h, ct.byref(st)))
+        self.nrhs = inc.nrhs
+
+    def assemble_rhs(self, eliminate=True) -> np.ndarray:
+        self._check(lib().lfasm_assemble_rhs(self._h, 1 if eliminate else 0))
+        re = np.empty(self.nrhs * self.n)
+        im = np.empty(self.nrhs * self.n)
+        self._check(lib().lfasm_download_rhs(self._h, _p(re), _p(im)))
+        return (re + 1j * im).reshape(self.nrhs, self.n)
+
+    def eps_sensitivity(self, elems, lam, u, rhs_col=-1) -> np.ndarray:
+        el = np.ascontiguousarray(np.asarray(elems, dtype=np.int32))
+        lam = np.ascontiguousarray(np.asarray(lam, dtype=np.complex128))
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.complex128))
+        out = np.empty(len(el), dtype=np.complex128)
+        self._check(lib().lfasm_eps_sensitivity(self._h, len(el), _p(el), _p(lam.view(np.float64)),
+                                                _p(u.view(np.float64)), rhs_col,
+                                                _p(out.view(np.float64))))
+        return out
 
     def timings(self) -> dict:
         ms = np.zeros(5)

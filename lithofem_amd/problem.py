@@ -31,6 +31,13 @@ class Region:
     is_pml: bool = False
     pml_top: bool = False
     name: str = ""
+    slab: int = 0               # background slab of the region
+    source: bool = False        # scattered-field source (frustum) region
+    eps_bg: complex | None = None   # background eps of the slab (source regions)
+
+    @property
+    def deps(self) -> complex:
+        return complex(self.eps) - complex(self.eps_bg) if self.source else 0j
 
 
 @dataclass
@@ -44,6 +51,7 @@ class Problem:
     regions: list
     pml_order: float = 2.0
     pml_target: float = 1e-8
+    slabs: list = field(default_factory=list)   # [(z_lo, z_hi, eps_bg)] bottom to top
     meta: dict = field(default_factory=dict)
 
     @property
@@ -122,10 +130,15 @@ def line_grating(nx=36, ny=4, nz_scale=1.0, theta=6.0, warp=0.3) -> Cell:
     mesh = structured_cell_mesh(xs, ys, np.array(zs), region_of, periodic=(True, True),
                                 warp_amp=warp, warp_z=(0.0, 120.0))
     k0, kx, ky = k_parallel(theta)
-    regions = [Region(EPS_CAP, True, False, "pml_bottom"), Region(EPS_CAP, name="capping"),
-               Region(1.0, name="vacuum"), Region(EPS_ABS, name="absorber"),
-               Region(1.0, name="vacuum_bg"), Region(1.0, True, True, "pml_top")]
-    pb = Problem(k0=k0, kx=kx, ky=ky, z_min=0.0, z_max=120.0, pml_thick=lam, regions=regions)
+    regions = [Region(EPS_CAP, True, False, "pml_bottom", slab=0),
+               Region(EPS_CAP, name="capping", slab=0),
+               Region(1.0, name="vacuum", slab=1),
+               Region(EPS_ABS, name="absorber", slab=2, source=True, eps_bg=1.0),
+               Region(1.0, name="vacuum_bg", slab=2),
+               Region(1.0, True, True, "pml_top", slab=2)]
+    slabs = [(0.0, 40.0, EPS_CAP), (40.0, 60.0, 1.0), (60.0, 120.0, 1.0)]
+    pb = Problem(k0=k0, kx=kx, ky=ky, z_min=0.0, z_max=120.0, pml_thick=lam, regions=regions,
+                 slabs=slabs)
     return Cell(mesh, pb, "line_grating")
 
 
@@ -161,10 +174,14 @@ def contact_hole(n_xy=12, nz_scale=1.0, side=48.0, theta=6.0, phi=0.0, warp=0.4,
     mesh = structured_cell_mesh(xs, xs.copy(), np.array(zs), region_of, periodic=(True, True),
                                 warp_amp=warp, warp_z=(0.0, 120.0))
     k0, kx, ky = k_parallel(theta, phi)
-    regions = [Region(EPS_ABS, True, False, "pml_bottom"), Region(EPS_ABS, name="absorber"),
-               Region(1.0, name="hole"), Region(1.0, name="vacuum"),
-               Region(1.0, True, True, "pml_top")]
-    pb = Problem(k0=k0, kx=kx, ky=ky, z_min=0.0, z_max=120.0, pml_thick=lam, regions=regions)
+    regions = [Region(EPS_ABS, True, False, "pml_bottom", slab=0),
+               Region(EPS_ABS, name="absorber", slab=0),
+               Region(1.0, name="hole", slab=0, source=True, eps_bg=EPS_ABS),
+               Region(1.0, name="vacuum", slab=1),
+               Region(1.0, True, True, "pml_top", slab=1)]
+    slabs = [(0.0, z_abs, EPS_ABS), (z_abs, 120.0, 1.0)]
+    pb = Problem(k0=k0, kx=kx, ky=ky, z_min=0.0, z_max=120.0, pml_thick=lam, regions=regions,
+                 slabs=slabs)
     return Cell(mesh, pb, "contact_hole")
 
 
@@ -198,8 +215,12 @@ def l_opening(n_xy=16, nz_scale=1.0, theta=6.0, warp=0.5) -> Cell:
     mesh = structured_cell_mesh(xs, xs.copy(), np.array(zs), region_of, periodic=(True, True),
                                 warp_amp=warp, warp_z=(0.0, 130.0))
     k0, kx, ky = k_parallel(theta)
-    regions = [Region(EPS_ABS, True, False, "pml_bottom"), Region(EPS_ABS, name="absorber"),
-               Region(1.0, name="opening"), Region(1.0, name="vacuum"),
-               Region(1.0, True, True, "pml_top")]
-    pb = Problem(k0=k0, kx=kx, ky=ky, z_min=0.0, z_max=130.0, pml_thick=lam, regions=regions)
+    regions = [Region(EPS_ABS, True, False, "pml_bottom", slab=0),
+               Region(EPS_ABS, name="absorber", slab=0),
+               Region(1.0, name="opening", slab=0, source=True, eps_bg=EPS_ABS),
+               Region(1.0, name="vacuum", slab=1),
+               Region(1.0, True, True, "pml_top", slab=1)]
+    slabs = [(0.0, 70.0, EPS_ABS), (70.0, 130.0, 1.0)]
+    pb = Problem(k0=k0, kx=kx, ky=ky, z_min=0.0, z_max=130.0, pml_thick=lam, regions=regions,
+                 slabs=slabs)
     return Cell(mesh, pb, "l_opening")

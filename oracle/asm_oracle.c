@@ -446,20 +446,117 @@ double orc_residual(int n, const int *Ip, const int *Jcol, const double *vre,
    return bn > 0.0 ? sqrt(rn / bn) : 0.0;
 }
 
-/* Per-element sensitivity lam_e^T (dA_e / d eps_e) u_e: the mass term is
- * linear in eps, so dA_e/d eps_e is the TE-transformed mass element matrix
- * with eps = 1; local vectors are sign * gathered global entries. */
+/* solver/problem_common.inc:219-236: incident envelope of one part */
+static void incident_at(const orc_incident *inc, int rhs, int slab, double z, C e[3])
+{
+   const double *w = inc->waves + 18 * ((size_t)rhs * inc->nslab + slab);
+   const C qA = ld(w + 2), qB = ld(w + 4);
+   const C up = cexp(IU * qA * (z - w[0]));
+   const C dn = cexp(IU * qB * (z - w[1]));
+   for (int i = 0; i < 3; ++i) { e[i] = ld(w + 6 + 2 * i) * up + ld(w + 12 + 2 * i) * dn; }
+}
+
+static void face_dual_vector(const orc_asm_data *ad, int e, C *b)
+{
+   if (ad->p < 2) { return; }
+   const int p = ad->p, foff = 6 * p, ntd = p * (p - 1);
+   const int *fo = ad->fo + (size_t)e * 4;
+   for (int f = 0; f < 4; ++f)
+   {
+      const double *Ti = orc_TInv + 4 * fo[f];
+      for (int i = 0; i < ntd / 2; ++i)
+      {
+         const int d0 = foff + f * ntd + 2 * i;
+         const C a = b[d0], c = b[d0 + 1];
+         b[d0] = Ti[0] * a + Ti[1] * c;
+         b[d0 + 1] = Ti[2] * a + Ti[3] * c;
+      }
+   }
+}
+
+/* ScatterSource (solver/problem_common.inc:238-266) integrated against the
+ * physical shapes on the mass rule, then TransformDual of the element
+ * vector. */
+void orc_rhs_element(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
+                     int e, int rhs, int unit_deps, double *b_out)
+{
+   const int nd = ad->nd;
+   C *b = (C *)b_out;
+   for (int k = 0; k < nd; ++k) { b[k] = 0.0; }
+   const int reg = ad->attr[e] - 1;
+   if (!inc->region_src[reg]) { return; }
+   const double *J = ad->J + (size_t)e * 9;
+   const double *Ji = ad->Jinv + (size_t)e * 9;
+   const double detJ = ad->detJ[e];
+   const double *v0 = ad->v0 + (size_t)e * 3;
+   C de = ld(inc->region_deps + 2 * reg);
+   if (ad->eps_elem)
+   {
+      /* per-element generalisation: eps_e - eps_bg, eps_bg = eps_reg - deps_reg */
+      de = ld(ad->eps_elem + 2 * (size_t)e) - (ld(pb->eps + 2 * reg) - de);
+   }
+   if (unit_deps) { de = 1.0; }
+   const C fac = pb->k0 * pb->k0 * de;
+   const orc_rule *rm = &ad->rule_mass;
+   for (int q = 0; q < rm->nq; ++q)
+   {
+      const double *xi = rm->ip + (size_t)q * 3;
+      const double z = v0[2] + J[6] * xi[0] + J[7] * xi[1] + J[8] * xi[2];
+      C ei[3];
+      incident_at(inc, rhs, inc->region_slab[reg], z, ei);
+      const double w = rm->w[q] * detJ;
+      const double *vs = rm->vshape + (size_t)q * nd * 3;
+      for (int k = 0; k < nd; ++k)
+      {
+         C s = 0.0;
+         for (int i = 0; i < 3; ++i)
+         {
+            const double v = vs[k * 3 + 0] * Ji[0 * 3 + i] + vs[k * 3 + 1] * Ji[1 * 3 + i] +
+                             vs[k * 3 + 2] * Ji[2 * 3 + i];
+            s += v * (fac * ei[i]);
+         }
+         b[k] += w * s;
+      }
+   }
+   face_dual_vector(ad, e, b);
+}
+
+void orc_rhs_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
+                 int rhs, double *b_out)
+{
+   const int nd = ad->nd;
+   C *b = (C *)b_out;
+   for (int i = 0; i < ad->height; ++i) { b[i] = 0.0; }
+   C *be = (C *)malloc(sizeof(C) * nd);
+   for (int e = 0; e < ad->ne; ++e)
+   {
+      orc_rhs_element(ad, pb, inc, e, rhs, 0, (double *)be);
+      const int *ed = ad->dof + (size_t)e * nd;
+      const double *es = ad->sign + (size_t)e * nd;
+      for (int a = 0; a < nd; ++a) { b[ed[a]] += es[a] * be[a]; }
+   }
+   free(be);
+}
+
+/* Per-element sensitivity lam^T (dA/d eps_e u - db/d eps_e): the mass term
+ * is linear in eps, so dA_e/d eps_e is the TE-transformed mass element
+ * matrix with eps = 1; db/d eps_e is the element RHS at unit eps - eps_bg;
+ * local vectors are sign * gathered global entries. */
 void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
+                         const orc_incident *inc, int rhs_col,
                          const double *lam, const double *u, double *g)
 {
    const int nd = ad->nd;
    C *Ae = (C *)malloc(sizeof(C) * nd * nd);
+   C *be = (C *)malloc(sizeof(C) * nd);
    double *pc = (double *)malloc(sizeof(double) * 3 * nd);
    double *pv = (double *)malloc(sizeof(double) * 3 * nd);
    const C one = 1.0;
    for (int e = 0; e < ad->ne; ++e)
    {
       reassemble_element_eps(ad, pb, e, ORC_MASS, 1, &one, Ae, pc, pv);
+      const int use_src = inc && rhs_col >= 0;
+      if (use_src) { orc_rhs_element(ad, pb, inc, e, rhs_col, 1, (double *)be); }
       const int *ed = ad->dof + (size_t)e * nd;
       const double *es = ad->sign + (size_t)e * nd;
       C acc = 0.0;
@@ -471,12 +568,14 @@ void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
          {
             row += Ae[(size_t)a * nd + b] * (es[b] * ld(u + 2 * (size_t)ed[b]));
          }
+         if (use_src) { row -= be[a]; }
          acc += la * row;
       }
       g[2 * (size_t)e] = creal(acc);
       g[2 * (size_t)e + 1] = cimag(acc);
    }
    free(Ae);
+   free(be);
    free(pc);
    free(pv);
 }

@@ -97,12 +97,36 @@ void orc_eliminate(int n, const int *I, const int *Jcol,
 double orc_residual(int n, const int *I, const int *Jcol, const double *vre,
                     const double *vim, const double *X2n, const double *B2n);
 
+/* Incident envelope per slab and source regions (solver/problem_common.inc:
+ * 11-16,218-266): waves [rhs][slab] x 18 doubles (zA, zB, qA, qB, A[3], B[3]
+ * complex interleaved); region_slab / region_deps / region_src per region. */
+typedef struct {
+   int nrhs, nslab;
+   const double *waves;
+   const int *region_slab;
+   const double *region_deps;
+   const int *region_src;
+} orc_incident;
+
+/* RHS element vector of incident field `rhs` (ScatterSource,
+ * solver/problem_common.inc:238-266) on the mass rule with the dual face
+ * transform (nd interleaved complex); unit_deps = 1 replaces the region's
+ * eps - eps_bg by 1 (the derivative with respect to eps).  (pinned) */
+void orc_rhs_element(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
+                     int e, int rhs, int unit_deps, double *b);
+
+/* Global RHS (sign scatter, ascending element order), n interleaved. */
+void orc_rhs_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
+                 int rhs, double *b);
+
 /* Per-element adjoint sensitivity (no reference counterpart; pinned by
- * central finite differences of the assembled matrix): for each element e,
- * g[e] = lam_e^T (dA_e/d eps_e) u_e, complex, where dA_e/d eps_e is the mass
- * term with eps = 1 and the local vectors are gathered from global complex
- * vectors with signs and the face-pair transform. */
+ * central finite differences of the assembled matrix and RHS): for each
+ * element e, g[e] = lam^T (dA/d eps_e u - db/d eps_e), complex, where
+ * dA/d eps_e is the mass term with eps = 1 and db/d eps_e the RHS of
+ * incident field rhs_col at unit eps - eps_bg (source regions only;
+ * inc == NULL or rhs_col < 0: matrix term only). */
 void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
+                         const orc_incident *inc, int rhs_col,
                          const double *lam, const double *u, double *g);
 
 #ifdef __cplusplus

@@ -47,6 +47,12 @@ class _AsmData(ct.Structure):
                 ("Jcol", ct.c_void_p), ("eps_elem", ct.c_void_p)]
 
 
+class _Incident(ct.Structure):
+    _fields_ = [("nrhs", ct.c_int), ("nslab", ct.c_int), ("waves", ct.c_void_p),
+                ("region_slab", ct.c_void_p), ("region_deps", ct.c_void_p),
+                ("region_src", ct.c_void_p)]
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data
 
@@ -77,7 +83,11 @@ def lib():
                                     ct.c_void_p, ct.c_void_p]
         L.orc_residual.argtypes = [ct.c_int] + [ct.c_void_p] * 6
         L.orc_residual.restype = ct.c_double
-        L.orc_eps_sensitivity.argtypes = [ct.c_void_p] * 5
+        L.orc_eps_sensitivity.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                          ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.orc_rhs_element.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                      ct.c_int, ct.c_void_p]
+        L.orc_rhs_all.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
         _lib = L
     return _lib
 
@@ -159,11 +169,34 @@ class Oracle:
             raise RuntimeError("CSR position missing")
         return vre, vim
 
-    def eps_sensitivity(self, lam: np.ndarray, u: np.ndarray) -> np.ndarray:
+    def set_incident(self, inc):
+        """inc: problem-local incident set (one problem): nrhs, nslab, waves
+        (nrhs*nslab, 18), region_slab, region_deps, region_src."""
+        keep = self._keep
+        w = np.ascontiguousarray(inc.waves, dtype=np.float64)
+        rs = np.ascontiguousarray(inc.region_slab, dtype=np.int32)
+        rd = np.ascontiguousarray(np.asarray(inc.region_deps, np.complex128).view(np.float64))
+        src = np.ascontiguousarray(inc.region_src, dtype=np.int32)
+        keep += [w, rs, rd, src]
+        self.inc = _Incident(inc.nrhs, inc.nslab, _ptr(w), _ptr(rs), _ptr(rd), _ptr(src))
+
+    def rhs(self, r: int) -> np.ndarray:
+        b = np.zeros(self.ad.height, dtype=np.complex128)
+        lib().orc_rhs_all(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), r, _ptr(b))
+        return b
+
+    def rhs_element(self, e: int, r: int, unit: bool = False) -> np.ndarray:
+        b = np.zeros(self.ad.nd, dtype=np.complex128)
+        lib().orc_rhs_element(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), e, r,
+                              1 if unit else 0, _ptr(b))
+        return b
+
+    def eps_sensitivity(self, lam: np.ndarray, u: np.ndarray, rhs_col: int = -1) -> np.ndarray:
         lam = np.ascontiguousarray(lam, dtype=np.complex128)
         u = np.ascontiguousarray(u, dtype=np.complex128)
         g = np.zeros(self.ad.ne, dtype=np.complex128)
-        lib().orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob),
+        inc = ct.byref(self.inc) if rhs_col >= 0 else None
+        lib().orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob), inc, rhs_col,
                                   _ptr(lam), _ptr(u), _ptr(g))
         return g
 

@@ -286,22 +286,65 @@ def test_csr_scatter_elimination_brute_force(p):
     assert abs(O.residual(I, J, ere, eim, X, B) - ref) < 1e-12 * ref
 
 
+def _inc(pb, fields):
+    from lithofem_amd.incident import incident_set
+    return incident_set([pb], [fields])
+
+
 def test_eps_sensitivity_matches_finite_differences():
     c, ad, I, J = case("grating", 2, 6.0, "tiny")
     pb = c.problem
+    inc = _inc(pb, [(6.0, 0.0, (0.0, 1.0))])
     rng = np.random.default_rng(11)
     eps0 = np.array([pb.regions[a - 1].eps for a in ad.attr], dtype=np.complex128)
     lam = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
     u = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
-    g = O.Oracle(ad, pb, I, J, eps_elem=eps0).eps_sensitivity(lam, u)
-    for e in rng.choice(ad.ne, 5, replace=False):
+    o0 = O.Oracle(ad, pb, I, J, eps_elem=eps0)
+    o0.set_incident(inc)
+    g = o0.eps_sensitivity(lam, u, rhs_col=0)
+    src = np.nonzero(ad.attr == 4)[0]            # absorber line: source region
+    bulk = np.nonzero(ad.attr == 2)[0]           # capping: no source
+    picks = list(rng.choice(src, 3, replace=False)) + list(rng.choice(bulk, 2, replace=False))
+    for e in picks:
         for dirn in (1.0, 1j):
             h = 1e-3
             vals = []
             for sgn in (1, -1):
                 ee = eps0.copy()
                 ee[e] += sgn * h * dirn
-                vre, vim = O.Oracle(ad, pb, I, J, eps_elem=ee).assemble()
-                vals.append(lam @ _spmv(I, J, vre + 1j * vim, u))
+                oe = O.Oracle(ad, pb, I, J, eps_elem=ee)
+                oe.set_incident(inc)
+                vre, vim = oe.assemble()
+                vals.append(lam @ (_spmv(I, J, vre + 1j * vim, u) - oe.rhs(0)))
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[e] * dirn) < 1e-8 * abs(g[e]), (e, dirn, fd, g[e])
+
+
+def test_rhs_constant_field_equals_mass_action():
+    """For a constant incident field E0 and source strength deps, the RHS is
+    b = -deps * M_src g(E0), with M_src the unit-eps mass matrix of the source
+    elements and g(E0) the DOF vector of E0 (exact in the discrete space)."""
+    from lithofem_amd.incident import IncidentSet
+    c, ad, I, J = case("hole", 2, 6.0, "tiny")
+    pb = c.problem
+    E0 = np.array([0.3 - 0.2j, 1.1 + 0.4j, -0.7 + 0.1j])
+    nslab = len(pb.slabs)
+    w = np.zeros((nslab, 18))
+    w[:, 6:12] = np.stack([E0.real, E0.imag], 1).ravel()      # A = E0, qA = 0
+    inc = IncidentSet(nrhs=1, nslab=nslab, waves=w,
+                      region_slab=np.array([r.slab for r in pb.regions], np.int32),
+                      region_deps=np.array([r.deps for r in pb.regions]),
+                      region_src=np.array([1 if r.source else 0 for r in pb.regions], np.int32))
+    orc = O.Oracle(ad, pb, I, J)
+    orc.set_incident(inc)
+    b = orc.rhs(0)
+    g = interpolate(ad, lambda X: np.tile(E0, (len(X), 1)))
+    src = ad.attr == 3
+    deps = pb.regions[2].deps
+    Ae = orc.elements(0, ad.ne, O.MASS, True)
+    Ae[~src] = 0.0
+    eps_e = np.array([pb.regions[a - 1].eps for a in ad.attr])
+    Ae[src] /= eps_e[src][:, None, None]            # unit-eps mass of the source elements
+    vre, vim = orc.scatter(Ae)
+    ref = -deps * _spmv(I, J, vre + 1j * vim, g)
+    assert maxnorm_err(b, ref) < 1e-13
