@@ -66,6 +66,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", type=int, default=0, help="local kernel selector")
     ap.add_argument("--nx", type=int, default=36)
+    ap.add_argument("--mode", default="assemble",
+                    choices=["assemble", "multimesh", "variants", "rhs", "adjoint"],
+                    help="assemble: the headline step; others: batched / adjoint workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
@@ -76,6 +79,13 @@ def main() -> int:
     from lithofem_amd import gpu
     from lithofem_amd.workload import (canonical_flops, line_grating_p3, local_bytes,
                                        scatter_bytes)
+
+    if args.mode != "assemble":
+        from lithofem_amd import benchmarks
+        out = getattr(benchmarks, args.mode)(log)
+        out.update({"n_gpus": 1, "dtype": "f64", "data": "synthetic", "mode": args.mode})
+        print(json.dumps(out), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     wl = line_grating_p3(nx=args.nx)

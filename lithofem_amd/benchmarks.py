@@ -1,0 +1,293 @@
+"""Secondary benchmark modes (bench.py --mode ...): batched assembly of many
+meshes, eps variants and incident fields, and the adjoint gradient.
+
+Each function returns a dict that bench.py prints as one JSON line.
+Workloads (all FP64 complex, synthetic structured meshes):
+
+* multimesh: 16 contact-hole cells, NE_i = round(5e4 * 8**u_i) tets
+  (u_i ~ U(0,1), seed 2), hole side ~ U(20, 60) nm, sidewall ~ U(80, 90) deg,
+  order 2, assembled as one block-diagonal batch vs one mesh at a time;
+* variants: 64 eps variants on the largest of those meshes (every absorber
+  element's eps perturbed by U(-5%, 5%) independently in real and imaginary
+  part, seed = variant id): delta assembly vs 64 sequential plain
+  assemblies with per-element eps;
+* rhs: 8 incidence angles (0..14 deg) x 2 polarisations on that mesh: the 8
+  matrices (one per k_par) as one batch, the 16 RHS in one launch;
+* adjoint: contact-hole cell at order 2 (8 x 8 x 19 hexahedra, 7,296 tets,
+  the host LU limits the size), gradient of the design objective with
+  respect to every absorber/hole element's eps, vs the forward assembly +
+  solve, with central differences on a few elements.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import gpu
+from .fespace import build_asm_data
+from .problem import contact_hole
+
+
+def hole_cell(ne_target, side, sidewall, theta=6.0, phi=0.0):
+    cubes = ne_target / 6.0
+    h = (96.0 * 96.0 * 147.0 / cubes) ** (1.0 / 3.0)
+    n_xy = max(3, int(round(96.0 / h)))
+    nz = cubes / (n_xy * n_xy)
+    return contact_hole(n_xy=n_xy, nz_scale=nz / 28.0, side=side, theta=theta, phi=phi,
+                        sidewall_deg=sidewall)
+
+
+def hole_family(n=16, seed=2):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0, 1, n)
+    sides = rng.uniform(20, 60, n)
+    angles = rng.uniform(80, 90, n)
+    return [hole_cell(5e4 * 8 ** ui, s, a) for ui, s, a in zip(u, sides, angles)]
+
+
+def _union_pattern(pats, heights):
+    Is, Js, off, nnz = [], [], 0, 0
+    for (I, J), n in zip(pats, heights):
+        Is.append(I[:-1].astype(np.int64) + nnz)
+        Js.append(J + off)
+        off += n
+        nnz += len(J)
+    I = np.concatenate(Is + [np.array([nnz])])
+    return I.astype(np.int32), np.concatenate(Js).astype(np.int32)
+
+
+def _sync_time(fn, reps=3):
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def multimesh(log) -> dict:
+    cells = hole_family()
+    ads = [build_asm_data(c.mesh, 2) for c in cells]
+    pats = [gpu.csr_symbolic(ad.dof, ad.height) for ad in ads]
+    ne = [ad.ne for ad in ads]
+    log(f"multimesh: {len(ads)} meshes, ne {min(ne)}..{max(ne)}, total {sum(ne)}")
+    # one mesh at a time (each its own handle, launches in sequence)
+    singles = []
+    for c, ad, (I, J) in zip(cells, ads, pats):
+        g = gpu.AsmGpuGlobal(0)
+        g.init(ad, c.problem)
+        g.build_global(I, J, ad.ess)
+        singles.append(g)
+
+    def run_singles():
+        for g in singles:
+            g.compute_local()
+            g.scatter_eliminate(True)
+
+    run_singles()
+    t_seq = _sync_time(run_singles)
+    ref = [g.download_values() for g in singles]
+    for g in singles:
+        g.close()
+    # one batch
+    b = gpu.AsmGpuGlobal(0)
+    b.init_batch([(ad, c.problem) for ad, c in zip(ads, cells)])
+    I, J = _union_pattern(pats, [ad.height for ad in ads])
+    offs = np.cumsum([0] + [ad.height for ad in ads[:-1]])
+    ess = np.concatenate([ad.ess + o for ad, o in zip(ads, offs)])
+    b.build_global(I, J, ess)
+
+    def run_batch():
+        b.compute_local()
+        b.scatter_eliminate(True)
+
+    run_batch()
+    t_bat = _sync_time(run_batch)
+    vre, vim = b.download_values()
+    pos = 0
+    same = True
+    for (rre, rim) in ref:
+        n = len(rre)
+        same = same and np.array_equal(vre[pos:pos + n], rre) and np.array_equal(vim[pos:pos + n], rim)
+        pos += n
+    tm = b.timings()
+    b.close()
+    return {"metric": "batched multi-mesh assembly speedup (16 uneven contact-hole meshes, p=2)",
+            "value": t_seq / t_bat, "unit": "x", "higher_is_better": True,
+            "detail": {"elements": sum(ne), "elements_per_mesh": ne, "seq_ms": t_seq * 1e3,
+                       "batch_ms": t_bat * 1e3, "batch_local_ms": tm["local_ms"],
+                       "batch_scatter_ms": tm["scatter_ms"],
+                       "elements_per_s": sum(ne) / t_bat, "bitwise_equal_to_single": same}}
+
+
+def variants(log, nvar=64) -> dict:
+    cells = hole_family()
+    big = int(np.argmax([c.mesh.ne for c in cells]))
+    cell = cells[big]
+    pb = cell.problem
+    ad = build_asm_data(cell.mesh, 2)
+    I, J = gpu.csr_symbolic(ad.dof, ad.height)
+    params = np.nonzero(ad.attr == 2)[0].astype(np.int32)      # absorber
+    eps_abs = pb.regions[1].eps
+    deps = np.empty((nvar, len(params)), np.complex128)
+    for v in range(nvar):
+        rng = np.random.default_rng(v)
+        deps[v] = (eps_abs.real * 0.05 * rng.uniform(-1, 1, len(params))
+                   + 1j * eps_abs.imag * 0.05 * rng.uniform(-1, 1, len(params)))
+    log(f"variants: ne={ad.ne} n={ad.height} nnz={len(J)} params={len(params)} nvar={nvar}")
+    g = gpu.AsmGpuGlobal(0)
+    g.init(ad, pb)
+    g.build_global(I, J, ad.ess)
+    eps0 = np.array([pb.regions[a - 1].eps for a in ad.attr], dtype=np.complex128)
+    ee = [eps0.copy() for _ in range(nvar)]
+    for v in range(nvar):
+        ee[v][params] += deps[v]
+
+    def sequential():
+        for v in range(nvar):
+            g.set_eps_elem(ee[v])
+            g.compute_local()
+            g.scatter_eliminate(True)
+
+    sequential()
+    t_seq = _sync_time(sequential, reps=2)
+
+    def batched():
+        g.set_eps_elem(None)
+        g.compute_local()
+        g.scatter_eliminate(True)
+        g.variants_prepare(params)
+        g.variants_assemble(deps, eliminate=True)
+
+    batched()
+    t_bat = _sync_time(batched, reps=2)
+    t_kernel = g.timings()["batch_ms"]
+    # spot check: variant 7 against a plain assembly
+    vr, vi = g.download_variant(7)
+    g.set_eps_elem(ee[7])
+    g.compute_local()
+    g.scatter_eliminate(True)
+    pr, pi = g.download_values()
+    err = float(np.max(np.abs((vr - pr) + 1j * (vi - pi))) / np.max(np.abs(pr + 1j * pi)))
+    g.close()
+    return {"metric": "64-variant eps batch speedup vs 64 sequential plain assemblies (p=2)",
+            "value": t_seq / t_bat, "unit": "x", "higher_is_better": True,
+            "detail": {"elements": ad.ne, "dof": ad.height, "nnz": len(J),
+                       "params": len(params), "variants": nvar,
+                       "sequential_ms": t_seq * 1e3, "batched_ms": t_bat * 1e3,
+                       "variant_kernel_ms": t_kernel, "variant7_maxnorm_vs_plain": err,
+                       "output_GB": nvar * len(J) * 16 / 1e9}}
+
+
+def rhs(log) -> dict:
+    from .incident import incident_set
+    cells = hole_family()
+    big = int(np.argmax([c.mesh.ne for c in cells]))
+    base = cells[big]
+    thetas = [2.0 * i for i in range(8)]
+    ad = build_asm_data(base.mesh, 2)
+    I, J = gpu.csr_symbolic(ad.dof, ad.height)
+    import copy
+    probs = []
+    for th in thetas:
+        pb = copy.deepcopy(base.problem)
+        k0 = pb.k0
+        pb.kx = k0 * math.sin(math.radians(th))
+        pb.ky = 0.0
+        probs.append(pb)
+    fields = [[(th, 0.0, (1.0, 0.0)), (th, 0.0, (0.0, 1.0))] for th in thetas]
+    inc = incident_set(probs, fields)
+    log(f"rhs: ne={ad.ne} x {len(thetas)} angles x 2 polarisations")
+    b = gpu.AsmGpuGlobal(0)
+    b.init_batch([(ad, pb) for pb in probs])
+    Iu, Ju = _union_pattern([(I, J)] * len(probs), [ad.height] * len(probs))
+    offs = np.arange(len(probs)) * ad.height
+    ess = np.concatenate([ad.ess + o for o in offs])
+    b.build_global(Iu, Ju, ess)
+    b.set_incident(inc)
+
+    def batched():
+        b.compute_local()
+        b.scatter_eliminate(True)
+        b.assemble_rhs(True)
+
+    batched()
+    t_bat = _sync_time(batched)
+    b.close()
+    # per angle: own handle, matrix + 2 RHS
+    singles = []
+    for pb, fl in zip(probs, fields):
+        g = gpu.AsmGpuGlobal(0)
+        g.init(ad, pb)
+        g.build_global(I, J, ad.ess)
+        g.set_incident(incident_set([pb], [fl]))
+        singles.append(g)
+
+    def seq():
+        for g in singles:
+            g.compute_local()
+            g.scatter_eliminate(True)
+            g.assemble_rhs(True)
+
+    seq()
+    t_seq = _sync_time(seq)
+    for g in singles:
+        g.close()
+    return {"metric": "8 angles x 2 polarisations: batched matrices + RHS speedup vs per angle (p=2)",
+            "value": t_seq / t_bat, "unit": "x", "higher_is_better": True,
+            "detail": {"elements_per_angle": ad.ne, "batched_ms": t_bat * 1e3,
+                       "sequential_ms": t_seq * 1e3}}
+
+
+def adjoint(log, n_xy=8, nz_scale=0.67, n_fd=2) -> dict:
+    import torch
+
+    from .adjoint import DesignObjective, EpsForwardModel, make_torch_function, plane_points
+    from .incident import evaluate, incident_set
+    cell = contact_hole(n_xy=n_xy, nz_scale=nz_scale, side=48.0, theta=6.0)
+    pb = cell.problem
+    ad = build_asm_data(cell.mesh, 2)
+    I, J = gpu.csr_symbolic(ad.dof, ad.height)
+    inc = incident_set([pb], [[(6.0, 0.0, (1.0, 0.0))]])
+    params = np.nonzero((ad.attr == 2) | (ad.attr == 3))[0].astype(np.int32)
+    log(f"adjoint: ne={ad.ne} n={ad.height} params={len(params)}")
+    model = EpsForwardModel(ad, pb, I, J, params, incident=inc, rhs_col=0)
+    rng = np.random.default_rng(3)
+    target = rng.uniform(0, 1, (32, 32))
+    pts = plane_points(96.0, 96.0, -1.6, 32, 32)
+    obj = DesignObjective(ad, pb, 96.0, 96.0, 115.0, -1.6, 32, target,
+                          evaluate(inc.waves, 0, pts[:, 2]), n_orders_grid=16)
+    eps0 = model.eps0[params].copy()
+    F = make_torch_function()
+    eps = torch.tensor(eps0, dtype=torch.complex128, requires_grad=True)
+    t0 = time.perf_counter()
+    u = F.apply(eps, model)
+    t_fwd = time.perf_counter() - t0
+    Jv = obj(u)
+    t0 = time.perf_counter()
+    Jv.backward()
+    t_bwd = time.perf_counter() - t0
+    grad = eps.grad.numpy()
+    rels = []
+    picks = np.random.default_rng(4).choice(len(params), n_fd, replace=False)
+    for i in picks:
+        h = 1e-6 * abs(eps0[i])
+        ep, em = eps0.copy(), eps0.copy()
+        ep[i] += h
+        em[i] -= h
+        jp = float(obj(torch.from_numpy(model.solve(ep))))
+        jm = float(obj(torch.from_numpy(model.solve(em))))
+        fd = (jp - jm) / (2 * h)
+        an = float(np.conj(grad[i]).real)
+        rels.append(abs(fd - an) / abs(an))
+    adj_ms = model.g.timings()["adjoint_ms"]
+    model.close()
+    return {"metric": "adjoint eps-gradient cost / forward assembly+solve (p=2)",
+            "value": t_bwd / t_fwd, "unit": "ratio", "higher_is_better": False,
+            "detail": {"elements": ad.ne, "dof": ad.height, "params": len(params),
+                       "forward_s": t_fwd, "backward_s": t_bwd,
+                       "sensitivity_kernel_ms": adj_ms, "objective": float(Jv.detach()),
+                       "fd_rel_errors": rels}}

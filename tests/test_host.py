@@ -1,0 +1,219 @@
+"""CPU tests of the C-ABI library's host side and of the input producers.
+
+* liblfasm.so loads and exports every function include/lithofem_asm.h
+  declares (no device call is made);
+* the host-only entry points (lfasm_csr_symbolic, lfasm_pml_smax) against
+  independent computations;
+* the integration rules integrate every monomial of their order exactly;
+* the Nedelec space built by lithofem_amd.fespace is H(curl)-conforming:
+  a random global DOF vector has the same tangential field on both sides
+  of every shared face, periodic faces included;
+* the layered-stack incident field satisfies the interface conditions
+  (tangential E and H, normal D) and the dispersion relation;
+* bench.py refuses more than one GPU.
+"""
+
+from __future__ import annotations
+
+import math
+import re
+import subprocess
+import sys
+from math import factorial
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from lithofem_amd import gpu
+from lithofem_amd.fespace import build_asm_data, csr_symbolic
+from lithofem_amd.incident import slab_waves
+from lithofem_amd.nedelec import FACES, ND_TINV, nd_tet
+from lithofem_amd.problem import contact_hole, line_grating
+from lithofem_amd.quadrature import monomial_exponents, tet_rule
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names.update(re.findall(r"\b(lfasm_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not gpu.LIB_PATH.exists():
+        pytest.fail("liblfasm.so is not built (run __graft_entry__.build())")
+    return gpu.lib()
+
+
+def test_library_exports_every_declared_function(lib):
+    names = _declared_functions()
+    assert len(names) >= 30, names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_library_csr_symbolic_equals_numpy(lib, p):
+    c = contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=0.0)
+    ad = build_asm_data(c.mesh, p)
+    I0, J0 = csr_symbolic(ad.dof, ad.height)
+    I1, J1 = gpu.csr_symbolic(ad.dof, ad.height)
+    assert np.array_equal(I0, I1) and np.array_equal(J0, J1)
+    # brute force: (i, j) is in the pattern iff some element holds both
+    rows = np.repeat(ad.dof, ad.nd, axis=1).ravel()
+    cols = np.tile(ad.dof, (1, ad.nd)).ravel()
+    keys = np.unique(rows.astype(np.int64) * ad.height + cols)
+    assert len(keys) == len(J1)
+    got = np.repeat(np.arange(ad.height), np.diff(I1)).astype(np.int64) * ad.height + J1
+    assert np.array_equal(np.sort(got), keys)
+
+
+def test_library_pml_smax(lib):
+    import ctypes as ct
+    f = lib.lfasm_pml_smax
+    f.restype = ct.c_double
+    f.argtypes = [ct.c_double] * 5
+    # -ln(1e-8) * 3 / (2 * 0.5 * 1.2 * 10) = 18.42068... * 3 / 12
+    assert abs(f(1e-8, 2.0, 0.5, 1.2, 10.0) - 8.0 * math.log(10.0) * 3.0 / 12.0) < 1e-14
+    pb = line_grating(nx=4, ny=3, nz_scale=0.15).problem
+    for top in (False, True):
+        ref = pb.smax(top)
+        got = f(pb.pml_target, pb.pml_order, pb.k0, pb.n_med(top), pb.pml_thick)
+        assert abs(got - ref) <= 1e-15 * abs(ref)
+
+
+@pytest.mark.parametrize("order", range(0, 9))
+def test_tet_rule_is_exact_to_its_order(order):
+    x, w = tet_rule(order)
+    assert np.all(x >= -1e-14) and np.all(x.sum(1) <= 1 + 1e-14)
+    for (i, j, k) in monomial_exponents(order):
+        exact = factorial(i) * factorial(j) * factorial(k) / factorial(i + j + k + 3)
+        got = float(np.sum(w * x[:, 0] ** i * x[:, 1] ** j * x[:, 2] ** k))
+        assert abs(got - exact) < 1e-14, (order, i, j, k, got, exact)
+
+
+def _field(ad, el, e, xi, g):
+    """Physical field of global DOF vector g inside element e at reference
+    point xi (the element transform: sign, then the face-pair block)."""
+    vs = el.vshape(xi[None, :])[0]                       # (nd, 3)
+    v = vs @ ad.Jinv.reshape(-1, 3, 3)[e]                # covariant map
+    loc = g[ad.dof[e]] * ad.sign[e]
+    if ad.p >= 2:
+        ntd = ad.p * (ad.p - 1)
+        Ti = ND_TINV.reshape(6, 4)
+        for f in range(4):
+            t = Ti[ad.fo[e, f]]
+            for i in range(ntd // 2):
+                d0 = 6 * ad.p + f * ntd + 2 * i
+                a, b = loc[d0], loc[d0 + 1]
+                loc[d0], loc[d0 + 1] = t[0] * a + t[2] * b, t[1] * a + t[3] * b
+    return loc @ v
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("name", ["grating", "hole"])
+def test_nedelec_space_is_tangentially_conforming(name, p):
+    c = line_grating(nx=4, ny=3, nz_scale=0.15, theta=6.0) if name == "grating" else \
+        contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=6.0)
+    mesh = c.mesh
+    ad = build_asm_data(mesh, p)
+    el = nd_tet(p)
+    rng = np.random.default_rng(11 + p)
+    g = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
+    owners = {}
+    for e in range(ad.ne):
+        for f, lv in enumerate(FACES):
+            key = tuple(sorted(int(mesh.elem_verts[e, v]) for v in lv))
+            owners.setdefault(key, []).append((e, lv))
+    shared = [v for v in owners.values() if len(v) == 2]
+    assert len(shared) > 100
+
+    def centroid(e, lv):
+        return mesh.elem_xyz[e][list(lv), :].mean(0)
+
+    periodic = [np.linalg.norm(centroid(*a) - centroid(*b)) > 1e-9 for a, b in shared]
+    per_idx = np.nonzero(periodic)[0]
+    int_idx = np.nonzero(~np.array(periodic))[0]
+    assert len(per_idx) > 0
+    picks = np.concatenate([rng.choice(per_idx, min(60, len(per_idx)), replace=False),
+                            rng.choice(int_idx, 60, replace=False)])
+    worst_t, worst_n = 0.0, 0.0
+    for s in picks:
+        (e1, lv1), (e2, lv2) = shared[s]
+        ids = sorted(int(mesh.elem_verts[e1, v]) for v in lv1)
+        wts = rng.dirichlet(np.ones(3))
+        fields = []
+        for e, lv in ((e1, lv1), (e2, lv2)):
+            lam = np.zeros(4)
+            for v in lv:
+                lam[v] = wts[ids.index(int(mesh.elem_verts[e, v]))]
+            fields.append(_field(ad, el, e, lam[1:], g))
+        xa, xb, xc = (mesh.elem_xyz[e1, v] for v in lv1)
+        normal = np.cross(xb - xa, xc - xa)
+        normal /= np.linalg.norm(normal)
+        d = fields[0] - fields[1]
+        ref = np.abs(fields[0]).max()
+        worst_t = max(worst_t, np.abs(d - (d @ normal) * normal).max() / ref)
+        worst_n = max(worst_n, abs(d @ normal) / ref)
+    assert worst_t < 1e-11, worst_t
+    # not the same element twice: the normal component does jump
+    assert worst_n > 1e-3, worst_n
+
+
+@pytest.mark.parametrize("phi", [0.0, 30.0])
+@pytest.mark.parametrize("jones", [(1.0, 0.0), (0.0, 1.0)])
+def test_incident_field_interface_conditions(phi, jones):
+    c = contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=9.0, phi=phi)
+    pb = c.problem
+    W = slab_waves(pb, 9.0, phi, jones)
+    slabs = pb.slabs
+    eps = [complex(s[2]) for s in slabs]
+    k0 = pb.k0
+    kpar = k0 * np.sqrt(eps[-1]).real * math.sin(math.radians(9.0))
+    kx, ky = kpar * math.cos(math.radians(phi)), kpar * math.sin(math.radians(phi))
+
+    def fields(j, z):
+        w = W[j]
+        qA, qB = w[2] + 1j * w[3], w[4] + 1j * w[5]
+        A = w[6:12:2] + 1j * w[7:12:2]
+        B = w[12:18:2] + 1j * w[13:18:2]
+        ea, eb = np.exp(1j * qA * (z - w[0])), np.exp(1j * qB * (z - w[1]))
+        E = ea * A + eb * B
+        dE = 1j * qA * ea * A + 1j * qB * eb * B
+        # curl of E exp(i k_par . r), envelope part
+        H = np.array([1j * ky * E[2] - dE[1], dE[0] - 1j * kx * E[2],
+                      1j * kx * E[1] - 1j * ky * E[0]])
+        div = 1j * kx * E[0] + 1j * ky * E[1] + dE[2]
+        return E, H, div, qA, qB, A, B
+
+    scale = np.abs(fields(len(slabs) - 1, slabs[-1][0])[0]).max()
+    for j in range(len(slabs)):
+        z = 0.5 * (slabs[j][0] + slabs[j][1])
+        E, H, div, qA, qB, A, B = fields(j, z)
+        assert abs(div) < 1e-12 * k0 * scale
+        for q in (qA, qB):
+            assert abs(q * q + kx * kx + ky * ky - k0 * k0 * eps[j]) < 1e-12 * k0 * k0
+    for j in range(1, len(slabs)):
+        z = slabs[j][0]
+        Eb, Hb, *_ = fields(j - 1, z)
+        Et, Ht, *_ = fields(j, z)
+        assert np.abs(Eb[:2] - Et[:2]).max() < 1e-12 * scale
+        assert abs(eps[j - 1] * Eb[2] - eps[j] * Et[2]) < 1e-12 * scale * abs(eps[j])
+        assert np.abs(Hb[:2] - Ht[:2]).max() < 1e-12 * k0 * scale
+    # nothing comes in from below: the bottom slab holds only the downward wave
+    assert np.abs(W[0][6:12]).max() < 1e-14
+    # unit incident amplitude from the top (|E_inc| = 1 for s, |H_inc| ~ n for p)
+    E_in = W[-1][12:18:2] + 1j * W[-1][13:18:2]
+    assert abs(np.linalg.norm(E_in) - 1.0) < 1e-12
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
