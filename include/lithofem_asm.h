@@ -149,6 +149,19 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
  * restores the built-in default.  No reference counterpart. */
 int lfasm_set_local_kernel(lfasm_handle *h, int kernel);
 
+/* Element-matrix path selection (no reference counterpart).  With the
+ * tensor path on (the default), elements whose four vertices lie in
+ * [z_min, z_max] of their problem -- Lambda = I at every quadrature point,
+ * so every coefficient is constant over the element -- take a
+ * reference-tensor kernel: Ae = sum of 21 precomputed reference-element
+ * tensors (built from the same rule tables) times element coefficients
+ * from J, eps and k_par; the other elements take the quadrature kernels
+ * that lfasm_compute_local_kernel's selector names.  Masks holding exactly
+ * one of LFASM_CROSS0 / LFASM_CROSS1 always use the quadrature kernels.
+ * Off: every element takes the quadrature kernels.  Returns the number of
+ * elements on the tensor path (0 when off). */
+int lfasm_set_tensor_path(lfasm_handle *h, int enable);
+
 /* AsmGpuGlobal::ScatterEliminate (solver/asm_gpu.inc:517-530): CSR values
  * from the local matrices by a deterministic per-row segmented reduction
  * (contributions summed in ascending element order), then (eliminate)
@@ -178,7 +191,9 @@ int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
 
 /* Device-event timings (milliseconds) of the last calls, in this order:
  * [0] compute_local kernels, [1] scatter_eliminate kernels,
- * [2] build_global device part, [3] last batched call, [4] last adjoint call. */
+ * [2] build_global device part, [3] last batched call, [4] last adjoint call,
+ * [5] reference-tensor part and [6] quadrature part of the last
+ * compute_local (0 when that part did not run). */
 int lfasm_timings(const lfasm_handle *h, double *ms, int n);
 
 /* ---- eps variants (no reference counterpart) ------------------------------

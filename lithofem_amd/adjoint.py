@@ -40,19 +40,30 @@ from .nedelec import ND_TINV, nd_tet
 class HostLU:
     """Sparse LU of the eliminated complex CSR system (SuperLU)."""
 
-    def __init__(self, I, J, vals, n):
-        A = sp.csr_matrix((vals, J, I), shape=(n, n)).tocsc()
+    def __init__(self, I, J, vals, n, refine=1):
+        self.A = sp.csr_matrix((vals, J, I), shape=(n, n))
         # the pattern is structurally symmetric: minimum degree on A^T + A
-        # with symmetric-mode (diagonal-preferring) pivoting keeps the 3-D
-        # fill far below a column ordering
-        self.lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.01,
+        # with diagonal pivots keeps the 3-D fill far below a column ordering
+        # (threshold pivoting off the diagonal breaks the symmetric ordering
+        # and multiplies the fill); iterative refinement against A restores
+        # the accuracy static pivoting gives away
+        self.lu = spla.splu(self.A.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
                             options=dict(SymmetricMode=True))
+        self.refine = refine
+
+    def _solve(self, b, trans):
+        b = np.asarray(b, dtype=np.complex128)
+        A = self.A if trans == "N" else self.A.T
+        x = self.lu.solve(b, trans=trans)
+        for _ in range(self.refine):
+            x = x + self.lu.solve(b - A @ x, trans=trans)
+        return x
 
     def solve(self, b):
-        return self.lu.solve(np.asarray(b, dtype=np.complex128))
+        return self._solve(b, "N")
 
     def solve_transpose(self, b):
-        return self.lu.solve(np.asarray(b, dtype=np.complex128), trans="T")
+        return self._solve(b, "T")
 
 
 # ---------------------------------------------------------------- sampling

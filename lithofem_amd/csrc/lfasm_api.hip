@@ -7,6 +7,7 @@
 #include "lfasm_device.hpp"
 #include "lfasm_peaks.hpp"
 #include "lfasm_batch.hpp"
+#include "lfasm_tensor.hpp"
 #include "../../include/lithofem_asm.h"
 
 #include <hip/hip_runtime.h>
@@ -29,7 +30,10 @@ struct DevRule {
    double *ip = nullptr, *w = nullptr, *vs = nullptr, *cs = nullptr;
 };
 
-enum { T_LOCAL = 0, T_SCATTER = 1, T_BUILD = 2, T_BATCH = 3, T_ADJ = 4, T_N = 5 };
+enum {
+   T_LOCAL = 0, T_SCATTER = 1, T_BUILD = 2, T_BATCH = 3, T_ADJ = 4,
+   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_N = 7
+};
 
 }  // namespace
 
@@ -53,6 +57,13 @@ struct lfasm_handle {
    double TInv[24];
    double *TInv_dev = nullptr;
    double2 *Ae = nullptr;
+   // constant-coefficient elements (every vertex between z_min and z_max of
+   // the element's problem) take the reference-tensor kernel; the others the
+   // quadrature kernels
+   int nconst = 0, nquad = 0;
+   int *clist = nullptr, *qlist = nullptr;
+   double *Rten = nullptr;
+   bool tensor_path = true;
    // global
    int64_t nnz = 0;
    int maxrow = 0;
@@ -122,6 +133,8 @@ void release_mesh(lfasm_handle *h)
    dfree(h->J); dfree(h->Ji); dfree(h->detJ); dfree(h->v0);
    dfree(h->attr); dfree(h->fo); dfree(h->dof); dfree(h->sign);
    dfree(h->eps_tab); dfree(h->eps_elem); dfree(h->TInv_dev); dfree(h->Ae);
+   dfree(h->clist); dfree(h->qlist); dfree(h->Rten);
+   h->nconst = h->nquad = 0;
    dfree(h->pml_tab); dfree(h->pid); dfree(h->eps_off);
    dfree(h->waves); dfree(h->region_slab); dfree(h->region_src); dfree(h->region_deps);
    h->inc_nrhs = 0;
@@ -201,6 +214,7 @@ LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_
    P.nout = h->ne;
    P.elist = nullptr;
    P.eps_unit = 0;
+   P.out_by_elem = 0;
    return P;
 }
 
@@ -235,7 +249,7 @@ hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
    e = hipGetLastError();
    if (e != hipSuccess || !P.apply_te) { return e; }
    hipLaunchKernelGGL(k_te, dim3(P.nout), dim3(64), 0, h->stream, P.nd, P.p, P.fo, P.elist,
-                      h->TInv_dev, P.Ae);
+                      P.out_by_elem, h->TInv_dev, P.Ae);
    return hipGetLastError();
 }
 
@@ -301,6 +315,41 @@ hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel)
    if (n2 <= 2048) { return launch_generic<4, 512>(h, P); }
    if (n2 <= 8192) { return launch_generic<8, 1024>(h, P); }
    return hipErrorInvalidValue;
+}
+
+// Reference-tensor kernel over the handle's constant-coefficient elements.
+hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
+{
+   TensorParams T;
+   T.nd = h->nd;
+   T.p = h->p;
+   T.npair = (h->nd + 1) / 2;
+   T.nb = T.npair * T.npair;
+   T.nlist = h->nconst;
+   T.elist = h->clist;
+   T.R = h->Rten;
+   T.J = h->J;
+   T.Ji = h->Ji;
+   T.detJ = h->detJ;
+   T.attr = h->attr;
+   T.fo = h->fo;
+   T.pid = h->pid;
+   T.eps_off = h->eps_off;
+   T.eps_tab = h->eps_tab;
+   T.eps_elem = h->eps_elem;
+   T.pml_tab = h->pml_tab;
+   std::memcpy(T.TInv, h->TInv, sizeof(T.TInv));
+   T.use_cc = P.use_cc;
+   T.use_mass = P.mflags & 1;
+   T.use_klk = (P.mflags >> 3) & 1;
+   T.use_cross = (P.mflags >> 1) & 1;
+   T.apply_te = P.apply_te;
+   T.Ae = P.Ae;
+   const int epar = T.nb > 128 ? 1 : 256 / T.nb;
+   const dim3 grid((unsigned)((h->nconst + TEN_EPC - 1) / TEN_EPC),
+                   epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
+   hipLaunchKernelGGL(k_local_tensor, grid, dim3(256), 0, h->stream, T, epar);
+   return hipGetLastError();
 }
 
 hipError_t ev_begin(lfasm_handle *h, int t)
@@ -559,6 +608,43 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
    std::memcpy(h->TInv, m0.TInv, sizeof(h->TInv));
    LF_HIP(upload(h, h->TInv_dev, h->TInv, 24));
    LF_HIP(hipMalloc((void **)&h->Ae, ne * nd * nd * sizeof(double2)));
+   // element classes: s(z) = 1 on the closed slab [z_min, z_max], so an
+   // element whose four vertices lie in it has Lambda = I at every
+   // quadrature point (the points are convex combinations of the vertices)
+   {
+      std::vector<int> cl, ql;
+      int64_t base = 0;
+      for (int i = 0; i < nmesh; ++i)
+      {
+         const lfasm_mesh &m = ms[i];
+         for (int e = 0; e < m.ne; ++e)
+         {
+            const double z0 = m.v0[(size_t)e * 3 + 2];
+            const double *Je = m.J + (size_t)e * 9;
+            bool inside = true;
+            for (int v = 0; v < 4; ++v)
+            {
+               const double z = v == 0 ? z0 : z0 + Je[6 + v - 1];
+               inside = inside && z >= m.pml.z_min && z <= m.pml.z_max;
+            }
+            (inside ? cl : ql).push_back((int)(base + e));
+         }
+         base += m.ne;
+      }
+      h->nconst = (int)cl.size();
+      h->nquad = (int)ql.size();
+      LF_HIP(upload(h, h->clist, cl.data(), cl.size()));
+      LF_HIP(upload(h, h->qlist, ql.data(), ql.size()));
+      const int npair = (m0.nd + 1) / 2;
+      const size_t nR = (size_t)npair * npair * NTENS * 4;
+      LF_HIP(hipMalloc((void **)&h->Rten, nR * sizeof(double)));
+      const RuleDev rc{h->rc.nq, h->rc.ip, h->rc.w, h->rc.vs, h->rc.cs};
+      const RuleDev rm{h->rm.nq, h->rm.ip, h->rm.w, h->rm.vs, h->rm.cs};
+      hipLaunchKernelGGL(k_ref_tensor, dim3((unsigned)std::min<size_t>((nR + 255) / 256, 4096)),
+                         dim3(256), 0, h->stream, m0.nd, npair, rc, rm, h->Rten);
+      LF_HIP(hipGetLastError());
+      LF_HIP(hipStreamSynchronize(h->stream));
+   }
    h->inited = true;
    return LFASM_OK;
 }
@@ -674,12 +760,33 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
    }
    LF_HIP(hipSetDevice(h->device));
    const LocalParams P = local_params(h, mask_cc, mask_mass, apply_te, h->Ae);
+   // the tensor kernel sums CROSS0 and CROSS1 as one antisymmetric term, so
+   // a mask holding only one of them goes through the quadrature kernels
+   const bool one_cross = ((P.mflags >> 1) & 1) != ((P.mflags >> 2) & 1);
+   const bool split = h->tensor_path && !one_cross && h->nconst > 0;
+   h->ev_used[T_LOCAL_TENSOR] = h->ev_used[T_LOCAL_QUAD] = false;
    LF_HIP(ev_begin(h, T_LOCAL));
    if (!P.use_cc && !P.mflags)
    {
       const size_t n = (size_t)h->ne * h->nd * h->nd;
       hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, h->Ae, n);
       LF_HIP(hipGetLastError());
+   }
+   else if (split)
+   {
+      LF_HIP(ev_begin(h, T_LOCAL_TENSOR));
+      LF_HIP(launch_tensor(h, P));
+      LF_HIP(ev_end(h, T_LOCAL_TENSOR));
+      if (h->nquad > 0)
+      {
+         LocalParams Pq = P;
+         Pq.elist = h->qlist;
+         Pq.nout = h->nquad;
+         Pq.out_by_elem = 1;
+         LF_HIP(ev_begin(h, T_LOCAL_QUAD));
+         LF_HIP(launch_local(h, Pq, kernel));
+         LF_HIP(ev_end(h, T_LOCAL_QUAD));
+      }
    }
    else
    {
@@ -700,6 +807,13 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel)
    }
    h->local_kernel = kernel;
    return LFASM_OK;
+}
+
+int lfasm_set_tensor_path(lfasm_handle *h, int enable)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   h->tensor_path = enable != 0;
+   return h->tensor_path ? h->nconst : 0;
 }
 
 int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te)

@@ -50,6 +50,7 @@ struct LocalParams {
    int nout;             // number of element matrices written (launch size)
    const int *elist;     // output i -> element elist[i] (NULL: element i)
    int eps_unit;         // 1: eps = 1 in the mass term (unit-eps mass matrices)
+   int out_by_elem;      // 1: write element elist[i]'s matrix to Ae[elist[i]], not Ae[i]
 };
 
 // ---- complex helpers ------------------------------------------------------
@@ -428,7 +429,7 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
    }
 
    if (!live) { return; }
-   double2 *Ae = P.Ae + (size_t)io * nd * nd;
+   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * nd;
 #pragma unroll
    for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -654,7 +655,7 @@ k_local_generic(LocalParams P, int nqt, int NP)
       }
       __syncthreads();
    }
-   double2 *Ae = P.Ae + (size_t)io * n2;
+   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * n2;
 #pragma unroll
    for (int i = 0; i < NACC; ++i)
    {
@@ -666,11 +667,11 @@ k_local_generic(LocalParams P, int nqt, int NP)
 // Dual face-DOF transform in place on Ae (solver/asm_gpu.inc:221-251):
 // one block per element, columns then rows.
 __global__ void k_te(int nd, int p, const int *__restrict__ fo, const int *__restrict__ elist,
-                     const double *TInvG, double2 *__restrict__ Ae)
+                     int out_by_elem, const double *TInvG, double2 *__restrict__ Ae)
 {
    const int io = blockIdx.x;
    const int e = elist ? elist[io] : io;
-   double2 *A = Ae + (size_t)io * nd * nd;
+   double2 *A = Ae + (size_t)(out_by_elem ? e : io) * nd * nd;
    const int foff = 6 * p;
    const int ntd = p * (p - 1);
    for (int axis = 0; axis < 2; ++axis)

@@ -348,10 +348,18 @@ class AsmGpuGlobal:
         return out
 
     def timings(self) -> dict:
-        ms = np.zeros(5)
-        self._check(lib().lfasm_timings(self._h, _p(ms), 5))
+        ms = np.zeros(7)
+        self._check(lib().lfasm_timings(self._h, _p(ms), 7))
         return dict(local_ms=ms[0], scatter_ms=ms[1], build_ms=ms[2], batch_ms=ms[3],
-                    adjoint_ms=ms[4])
+                    adjoint_ms=ms[4], local_tensor_ms=ms[5], local_quad_ms=ms[6])
+
+    def set_tensor_path(self, enable: bool) -> int:
+        """Reference-tensor kernel for constant-coefficient elements (on by
+        default); returns the number of elements it covers."""
+        rc = lib().lfasm_set_tensor_path(self._h, 1 if enable else 0)
+        if rc < 0:
+            raise AsmGpuError(rc, self.last_error())
+        return rc
 
     def stream(self) -> int:
         return lib().lfasm_stream(self._h) or 0

@@ -37,6 +37,13 @@ def local_bytes(ne: int, nd: int) -> float:
     return float(ne) * (9 * 8 * 2 + 8 + 24 + 4 + 16 + 16.0 * nd * nd)
 
 
+def tensor_bytes(ne: int, nd: int) -> float:
+    """Compulsory HBM bytes of the reference-tensor kernel per launch: per
+    element the list entry, J, Jinv, detJ, attr, fo and the Ae write (the
+    reference tensors and region table stay in cache)."""
+    return float(ne) * (4 + 9 * 8 * 2 + 8 + 4 + 16 + 16.0 * nd * nd)
+
+
 def scatter_bytes(ne: int, nd: int, n: int, nnz: int) -> float:
     return float(ne) * nd * nd * (16 + 2) + 16.0 * nnz + 4.0 * (n + 1) + \
         float(ne) * nd * (4 + 4 + 8) + 4.0 * (n + 1)

@@ -50,7 +50,13 @@ U3 = [("curlcurl", gpu.CURLCURL, 0, O.CURLCURL, False),
       ("mass", 0, gpu.MASS, O.MASS, False),
       ("cross0", 0, gpu.CROSS0, O.CROSS0, True),
       ("cross1", 0, gpu.CROSS1, O.CROSS1, True),
+      ("cross01", 0, gpu.CROSS0 | gpu.CROSS1, O.CROSS0 | O.CROSS1, True),
       ("klk", 0, gpu.KLK, O.KLK, True)]
+
+# (tensor path, quadrature kernels): with the tensor path on, constant-
+# coefficient elements take the reference-tensor kernel and the PML
+# elements the selected quadrature kernel; off, every element takes it
+PATHS = [(True, [gpu.KERNEL_AUTO, gpu.KERNEL_VALU]), (False, KERNELS)]
 
 
 @pytest.mark.parametrize("theta", [0.0, 6.0], ids=["t0", "t6"])
@@ -69,21 +75,24 @@ def test_u3_u4_local(name, p, theta):
     ref_full = np.stack([orc.element(e, full, True) for e in sample])
     g = _ctx(c, ad)
     try:
-        for kern in KERNELS:
-            # U3: each integrator alone, no face transform
-            for nm, mcc, mm, om, need_kpar in U3:
-                if need_kpar and not kpar:
-                    continue
-                g.compute_local(mcc, mm, False, kernel=kern)
+        for tensor, kernels in PATHS:
+            nconst = g.set_tensor_path(tensor)
+            assert (nconst > 0) == tensor
+            for kern in kernels:
+                # U3: each integrator alone, no face transform
+                for nm, mcc, mm, om, need_kpar in U3:
+                    if need_kpar and not kpar:
+                        continue
+                    g.compute_local(mcc, mm, False, kernel=kern)
+                    Ag = g.download_local()
+                    err = maxnorm_err(Ag[sample], refs[nm])
+                    assert err < TOL, (tensor, kern, nm, err)
+                # U4: full sum + face transform
+                g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=kern)
                 Ag = g.download_local()
-                err = maxnorm_err(Ag[sample], refs[nm])
-                assert err < TOL, (kern, nm, err)
-            # U4: full sum + face transform
-            g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=kern)
-            Ag = g.download_local()
-            err = maxnorm_err(Ag[sample], ref_full)
-            print(f"  kernel {kern}: u4 {err:.2e}", flush=True)
-            assert err < TOL, (kern, "u4", err)
+                err = maxnorm_err(Ag[sample], ref_full)
+                print(f"  tensor {tensor} kernel {kern}: u4 {err:.2e}", flush=True)
+                assert err < TOL, (tensor, kern, "u4", err)
     finally:
         g.close()
 
@@ -133,6 +142,17 @@ def test_u5_u6_global(name, p, theta):
         rg = g.residual(X, B)
         ro = O.residual(I, J, ere, eim, X, B)
         assert abs(rg - ro) <= 1e-12 * ro
+        # the tensor / quadrature split against quadrature everywhere
+        t = g.timings()
+        assert t["local_tensor_ms"] > 0 and t["local_quad_ms"] > 0
+        assert g.set_tensor_path(False) == 0
+        g.compute_local()
+        g.scatter_eliminate(True)
+        qre, qim = g.download_values()
+        assert g.timings()["local_tensor_ms"] == 0
+        errq = maxnorm_err(ere + 1j * eim, qre + 1j * qim)
+        print(f"  tensor path vs quadrature only {errq:.2e}", flush=True)
+        assert errq < TOL, errq
     finally:
         g.close()
 
