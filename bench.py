@@ -69,8 +69,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the quadrature-only comparison launches (clean kernel traces)")
     ap.add_argument("--kernel", type=int, default=0, help="local kernel selector")
     ap.add_argument("--nx", type=int, default=36)
+    ap.add_argument("--graph", default="on", choices=["on", "off"],
+                    help="time the step as a captured HIP graph (on) or eager calls (off)")
     ap.add_argument("--tensor", default="on", choices=["on", "off"],
                     help="reference-tensor kernel for constant-coefficient elements")
     ap.add_argument("--mode", default="assemble",
@@ -140,14 +144,28 @@ def main() -> int:
         tsum += step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_step = wall / args.steps * 1e3
+    ms_eager = wall / args.steps * 1e3
     tl_avg, ts_avg, tt_avg, tq_avg = tsum / args.steps
+    log(f"eager step {ms_eager:.3f} ms")
+    ms_step = ms_eager
+    if args.graph == "on":
+        # the same step as one captured HIP graph per launch
+        g.step_capture(gpu.CURLCURL, gpu.MASS_ALL, True, True)
+        for i in range(args.warmup):
+            g.step_launch(sync=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            g.step_launch(sync=False)
+        torch.cuda.synchronize()       # device-wide: waits for the library's streams too
+        ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+        log(f"graph step {ms_step:.3f} ms")
     log(f"step {ms_step:.3f} ms (element matrices {tl_avg:.3f} ms = tensor {tt_avg:.3f} + "
         f"quadrature {tq_avg:.3f}; scatter {ts_avg:.3f} ms)")
 
     # the same element matrices with the quadrature kernels on every element
     quad_only_ms = None
-    if nconst > 0:
+    if nconst > 0 and not args.no_compare:
         g.set_tensor_path(False)
         qs = []
         for _ in range(3):
@@ -201,6 +219,8 @@ def main() -> int:
                      "unit": dom["unit"], "frac": dom["frac"], "traffic": None,
                      "kernel": dom_name},
         "detail": {"element_matrices_ms": tl_avg, "scatter_ms": ts_avg,
+                   "step_ms_eager": ms_eager, "step_launch": "hip graph" if args.graph == "on"
+                   else "eager",
                    "kernels": kernels,
                    "elements_tensor_path": nconst, "elements_quadrature": nquad,
                    "element_matrices_quadrature_only_ms": quad_only_ms,

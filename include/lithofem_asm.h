@@ -162,6 +162,18 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel);
  * elements on the tensor path (0 when off). */
 int lfasm_set_tensor_path(lfasm_handle *h, int enable);
 
+/* Captured assembly step (no reference counterpart; the reference launches
+ * ComputeLocal and ScatterEliminate one kernel at a time,
+ * solver/lithofem_solve.cpp:789-790).  capture records the whole step --
+ * element matrices (both kernel paths) and scatter + elimination -- into a
+ * HIP graph on the handle's stream; launch replays it (sync = 1 waits for
+ * it).  Calls that change
+ * buffers or launch choices (init, build_global, set_eps_elem,
+ * set_local_kernel, set_tensor_path) drop the graph, and the next launch
+ * captures it again with the same masks.  The step holds no memset nodes. */
+int lfasm_step_capture(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te, int eliminate);
+int lfasm_step_launch(lfasm_handle *h, int sync);
+
 /* AsmGpuGlobal::ScatterEliminate (solver/asm_gpu.inc:517-530): CSR values
  * from the local matrices by a deterministic per-row segmented reduction
  * (contributions summed in ascending element order), then (eliminate)
@@ -193,7 +205,7 @@ int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
  * [0] compute_local kernels, [1] scatter_eliminate kernels,
  * [2] build_global device part, [3] last batched call, [4] last adjoint call,
  * [5] reference-tensor part and [6] quadrature part of the last
- * compute_local (0 when that part did not run). */
+ * compute_local (0 when that part did not run), [7] last lfasm_step_launch. */
 int lfasm_timings(const lfasm_handle *h, double *ms, int n);
 
 /* ---- eps variants (no reference counterpart) ------------------------------

@@ -32,7 +32,7 @@ struct DevRule {
 
 enum {
    T_LOCAL = 0, T_SCATTER = 1, T_BUILD = 2, T_BATCH = 3, T_ADJ = 4,
-   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_N = 7
+   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_STEP = 7, T_N = 8
 };
 
 }  // namespace
@@ -40,6 +40,12 @@ enum {
 struct lfasm_handle {
    int device = 0;
    hipStream_t stream = nullptr;
+   // captured assembly step (element matrices + scatter)
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t graph_exec = nullptr;
+   bool graph_dirty = true;
+   bool g_have = false;                       // capture parameters set
+   int g_mask_cc = 0, g_mask_mass = 0, g_apply_te = 0, g_elim = 0;
    std::string err;
    // mesh
    int ne = 0, nd = 0, p = 0, height = 0, nreg = 0;
@@ -128,8 +134,20 @@ void dfree(T *&p)
    if (p) { (void)hipFree((void *)p); p = nullptr; }
 }
 
+// The captured step holds buffer addresses and launch choices: every call
+// that changes either drops it (the next lfasm_step_launch captures anew).
+void drop_graph(lfasm_handle *h)
+{
+   // a launched step may still read the buffers the caller is about to change
+   if (h->graph_exec && h->stream) { (void)hipStreamSynchronize(h->stream); }
+   if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+   if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+   h->graph_dirty = true;
+}
+
 void release_mesh(lfasm_handle *h)
 {
+   drop_graph(h);
    dfree(h->J); dfree(h->Ji); dfree(h->detJ); dfree(h->v0);
    dfree(h->attr); dfree(h->fo); dfree(h->dof); dfree(h->sign);
    dfree(h->eps_tab); dfree(h->eps_elem); dfree(h->TInv_dev); dfree(h->Ae);
@@ -149,6 +167,7 @@ void release_mesh(lfasm_handle *h)
 
 void release_global(lfasm_handle *h)
 {
+   drop_graph(h);
    dfree(h->Ip); dfree(h->Jc); dfree(h->rc_ptr); dfree(h->rc_list);
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
@@ -360,6 +379,86 @@ hipError_t ev_begin(lfasm_handle *h, int t)
 
 hipError_t ev_end(lfasm_handle *h, int t) { return hipEventRecord(h->ev[2 * t + 1], h->stream); }
 
+#define LF_TRY(call)                                                             \
+   do {                                                                          \
+      hipError_t e_ = (call);                                                    \
+      if (e_ != hipSuccess) { return e_; }                                       \
+   } while (0)
+
+// Element matrices of the whole mesh, enqueued on h->stream without a
+// host wait: with the tensor path, the reference-tensor kernel over the
+// constant-coefficient elements, then the quadrature kernel over the PML
+// elements (both fill the device, so running them on two streams gains
+// nothing).  `timed` records the timing events (never inside a graph
+// capture).
+hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool timed)
+{
+   // the tensor kernel sums CROSS0 and CROSS1 as one antisymmetric term, so
+   // a mask holding only one of them goes through the quadrature kernels
+   const bool one_cross = ((P.mflags >> 1) & 1) != ((P.mflags >> 2) & 1);
+   const bool split = h->tensor_path && !one_cross && h->nconst > 0;
+   if (timed)
+   {
+      h->ev_used[T_LOCAL_TENSOR] = h->ev_used[T_LOCAL_QUAD] = false;
+      LF_TRY(ev_begin(h, T_LOCAL));
+   }
+   if (!P.use_cc && !P.mflags)
+   {
+      const size_t n = (size_t)h->ne * h->nd * h->nd;
+      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, h->Ae, n);
+      LF_TRY(hipGetLastError());
+   }
+   else if (split)
+   {
+      if (timed) { LF_TRY(ev_begin(h, T_LOCAL_TENSOR)); }
+      LF_TRY(launch_tensor(h, P));
+      if (timed) { LF_TRY(ev_end(h, T_LOCAL_TENSOR)); }
+      if (h->nquad > 0)
+      {
+         LocalParams Pq = P;
+         Pq.elist = h->qlist;
+         Pq.nout = h->nquad;
+         Pq.out_by_elem = 1;
+         if (timed) { LF_TRY(ev_begin(h, T_LOCAL_QUAD)); }
+         LF_TRY(launch_local(h, Pq, kernel));
+         if (timed) { LF_TRY(ev_end(h, T_LOCAL_QUAD)); }
+      }
+   }
+   else
+   {
+      LF_TRY(launch_local(h, P, kernel));
+   }
+   if (timed) { LF_TRY(ev_end(h, T_LOCAL)); }
+   return hipSuccess;
+}
+
+// Scatter + elimination of the element matrices into the CSR values.
+hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
+{
+   ScatterParams S;
+   S.n = h->height;
+   S.nd = h->nd;
+   S.Ip = h->Ip;
+   S.Jc = h->Jc;
+   S.eflag = h->eflag;
+   S.rc_ptr = h->rc_ptr;
+   S.rc_list = h->rc_list;
+   S.slot = h->slot;
+   S.is_ess = h->ess;
+   S.Ae = h->Ae;
+   S.vals = h->vals;
+   S.eliminate = eliminate ? 1 : 0;
+   S.maxrow = h->maxrow;
+   const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
+   LF_TRY(hipFuncSetAttribute((const void *)k_scatter_rows,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+   if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
+   hipLaunchKernelGGL(k_scatter_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem, h->stream, S);
+   LF_TRY(hipGetLastError());
+   if (timed) { LF_TRY(ev_end(h, T_SCATTER)); }
+   return hipSuccess;
+}
+
 }  // namespace
 
 // ---- C-ABI -------------------------------------------------------------------
@@ -386,6 +485,7 @@ int lfasm_create(int device, lfasm_handle **out)
    *out = h;
    LF_HIP(hipSetDevice(device));
    LF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
    for (int i = 0; i < 2 * T_N; ++i) { LF_HIP(hipEventCreate(&h->ev[i])); }
    return LFASM_OK;
 }
@@ -401,6 +501,7 @@ void lfasm_destroy(lfasm_handle *h)
    {
       if (h->ev[i]) { (void)hipEventDestroy(h->ev[i]); }
    }
+   drop_graph(h);
    if (h->stream) { (void)hipStreamDestroy(h->stream); }
    delete h;
 }
@@ -666,6 +767,7 @@ int lfasm_set_eps_elem(lfasm_handle *h, const double *eps)
 {
    if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
    LF_HIP(hipSetDevice(h->device));
+   drop_graph(h);
    if (!eps)
    {
       dfree(h->eps_elem);
@@ -760,39 +862,7 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
    }
    LF_HIP(hipSetDevice(h->device));
    const LocalParams P = local_params(h, mask_cc, mask_mass, apply_te, h->Ae);
-   // the tensor kernel sums CROSS0 and CROSS1 as one antisymmetric term, so
-   // a mask holding only one of them goes through the quadrature kernels
-   const bool one_cross = ((P.mflags >> 1) & 1) != ((P.mflags >> 2) & 1);
-   const bool split = h->tensor_path && !one_cross && h->nconst > 0;
-   h->ev_used[T_LOCAL_TENSOR] = h->ev_used[T_LOCAL_QUAD] = false;
-   LF_HIP(ev_begin(h, T_LOCAL));
-   if (!P.use_cc && !P.mflags)
-   {
-      const size_t n = (size_t)h->ne * h->nd * h->nd;
-      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, h->Ae, n);
-      LF_HIP(hipGetLastError());
-   }
-   else if (split)
-   {
-      LF_HIP(ev_begin(h, T_LOCAL_TENSOR));
-      LF_HIP(launch_tensor(h, P));
-      LF_HIP(ev_end(h, T_LOCAL_TENSOR));
-      if (h->nquad > 0)
-      {
-         LocalParams Pq = P;
-         Pq.elist = h->qlist;
-         Pq.nout = h->nquad;
-         Pq.out_by_elem = 1;
-         LF_HIP(ev_begin(h, T_LOCAL_QUAD));
-         LF_HIP(launch_local(h, Pq, kernel));
-         LF_HIP(ev_end(h, T_LOCAL_QUAD));
-      }
-   }
-   else
-   {
-      LF_HIP(launch_local(h, P, kernel));
-   }
-   LF_HIP(ev_end(h, T_LOCAL));
+   LF_HIP(enqueue_local(h, P, kernel, true));
    LF_HIP(hipStreamSynchronize(h->stream));
    h->local_valid = true;
    return LFASM_OK;
@@ -806,6 +876,7 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel)
       return h->fail(LFASM_E_ARG, "kernel selector %d outside [0, %d)", kernel, 2 + N_MFMA_VARIANTS);
    }
    h->local_kernel = kernel;
+   drop_graph(h);
    return LFASM_OK;
 }
 
@@ -813,7 +884,58 @@ int lfasm_set_tensor_path(lfasm_handle *h, int enable)
 {
    if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
    h->tensor_path = enable != 0;
+   drop_graph(h);
    return h->tensor_path ? h->nconst : 0;
+}
+
+int lfasm_step_capture(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te, int eliminate)
+{
+   if (!h || !h->global_built)
+   {
+      return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE;
+   }
+   LF_HIP(hipSetDevice(h->device));
+   drop_graph(h);
+   h->g_have = true;
+   h->g_mask_cc = mask_cc;
+   h->g_mask_mass = mask_mass;
+   h->g_apply_te = apply_te;
+   h->g_elim = eliminate;
+   const LocalParams P = local_params(h, mask_cc, mask_mass, apply_te, h->Ae);
+   LF_HIP(hipStreamSynchronize(h->stream));
+   LF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+   hipError_t e = enqueue_local(h, P, 0, false);
+   if (e == hipSuccess) { e = enqueue_scatter(h, eliminate, false); }
+   hipGraph_t g = nullptr;
+   const hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+   if (e == hipSuccess) { e = e2; }
+   if (e == hipSuccess) { h->graph = g; }
+   else if (g) { (void)hipGraphDestroy(g); }
+   LF_HIP(e);
+   LF_HIP(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+   h->graph_dirty = false;
+   return LFASM_OK;
+}
+
+int lfasm_step_launch(lfasm_handle *h, int sync)
+{
+   if (!h || !h->global_built)
+   {
+      return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE;
+   }
+   if (!h->g_have) { return h->fail(LFASM_E_STATE, "lfasm_step_capture first"); }
+   LF_HIP(hipSetDevice(h->device));
+   if (h->graph_dirty)
+   {
+      const int rc = lfasm_step_capture(h, h->g_mask_cc, h->g_mask_mass, h->g_apply_te, h->g_elim);
+      if (rc != LFASM_OK) { return rc; }
+   }
+   LF_HIP(ev_begin(h, T_STEP));
+   LF_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+   LF_HIP(ev_end(h, T_STEP));
+   h->local_valid = true;
+   if (sync) { LF_HIP(hipStreamSynchronize(h->stream)); }
+   return LFASM_OK;
 }
 
 int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te)
@@ -828,27 +950,7 @@ int lfasm_scatter_eliminate(lfasm_handle *h, int eliminate)
       return h ? h->fail(LFASM_E_STATE, "build_global and compute_local first") : LFASM_E_STATE;
    }
    LF_HIP(hipSetDevice(h->device));
-   ScatterParams S;
-   S.n = h->height;
-   S.nd = h->nd;
-   S.Ip = h->Ip;
-   S.Jc = h->Jc;
-   S.eflag = h->eflag;
-   S.rc_ptr = h->rc_ptr;
-   S.rc_list = h->rc_list;
-   S.slot = h->slot;
-   S.is_ess = h->ess;
-   S.Ae = h->Ae;
-   S.vals = h->vals;
-   S.eliminate = eliminate ? 1 : 0;
-   S.maxrow = h->maxrow;
-   const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
-   LF_HIP(hipFuncSetAttribute((const void *)k_scatter_rows,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-   LF_HIP(ev_begin(h, T_SCATTER));
-   hipLaunchKernelGGL(k_scatter_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem, h->stream, S);
-   LF_HIP(hipGetLastError());
-   LF_HIP(ev_end(h, T_SCATTER));
+   LF_HIP(enqueue_scatter(h, eliminate, true));
    LF_HIP(hipStreamSynchronize(h->stream));
    return LFASM_OK;
 }

@@ -348,10 +348,19 @@ class AsmGpuGlobal:
         return out
 
     def timings(self) -> dict:
-        ms = np.zeros(7)
-        self._check(lib().lfasm_timings(self._h, _p(ms), 7))
+        ms = np.zeros(8)
+        self._check(lib().lfasm_timings(self._h, _p(ms), 8))
         return dict(local_ms=ms[0], scatter_ms=ms[1], build_ms=ms[2], batch_ms=ms[3],
-                    adjoint_ms=ms[4], local_tensor_ms=ms[5], local_quad_ms=ms[6])
+                    adjoint_ms=ms[4], local_tensor_ms=ms[5], local_quad_ms=ms[6],
+                    step_ms=ms[7])
+
+    # -- captured step (HIP graph)
+    def step_capture(self, mask_cc=CURLCURL, mask_mass=MASS_ALL, apply_te=True, eliminate=True):
+        self._check(lib().lfasm_step_capture(self._h, mask_cc, mask_mass, 1 if apply_te else 0,
+                                             1 if eliminate else 0))
+
+    def step_launch(self, sync=True):
+        self._check(lib().lfasm_step_launch(self._h, 1 if sync else 0))
 
     def set_tensor_path(self, enable: bool) -> int:
         """Reference-tensor kernel for constant-coefficient elements (on by
