@@ -75,8 +75,9 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=36)
     ap.add_argument("--graph", default="on", choices=["on", "off"],
                     help="time the step as a captured HIP graph (on) or eager calls (off)")
-    ap.add_argument("--tensor", default="on", choices=["on", "off"],
-                    help="reference-tensor kernel for constant-coefficient elements")
+    ap.add_argument("--tensor", default="on", choices=["on", "off", "fused"],
+                    help="constant-coefficient elements: reference-tensor kernel (on), "
+                         "quadrature kernels (off), tensor rows inside the scatter (fused)")
     ap.add_argument("--mode", default="assemble",
                     choices=["assemble", "multimesh", "variants", "rhs", "adjoint"],
                     help="assemble: the headline step; others: batched / adjoint workloads")
@@ -115,7 +116,7 @@ def main() -> int:
     g.init(ad, pb)
     g.build_global(I, J, ad.ess)
     t_upload = time.perf_counter() - t0
-    nconst = g.set_tensor_path(args.tensor == "on")
+    nconst = g.set_tensor_path({"on": 1, "off": 0, "fused": 2}[args.tensor])
     nquad = ad.ne - nconst
     log(f"element classes: {nconst} constant-coefficient (reference-tensor kernel), "
         f"{nquad} PML (quadrature kernel)")
@@ -172,7 +173,7 @@ def main() -> int:
             g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=args.kernel)
             qs.append(g.timings()["local_ms"])
         quad_only_ms = min(qs)
-        g.set_tensor_path(True)
+        g.set_tensor_path({"on": 1, "off": 0, "fused": 2}[args.tensor])
         g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=args.kernel)
         log(f"quadrature kernels on all elements: {quad_only_ms:.3f} ms")
 

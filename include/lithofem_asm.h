@@ -149,18 +149,25 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
  * restores the built-in default.  No reference counterpart. */
 int lfasm_set_local_kernel(lfasm_handle *h, int kernel);
 
-/* Element-matrix path selection (no reference counterpart).  With the
- * tensor path on (the default), elements whose four vertices lie in
- * [z_min, z_max] of their problem -- Lambda = I at every quadrature point,
- * so every coefficient is constant over the element -- take a
- * reference-tensor kernel: Ae = sum of 21 precomputed reference-element
- * tensors (built from the same rule tables) times element coefficients
- * from J, eps and k_par; the other elements take the quadrature kernels
- * that lfasm_compute_local_kernel's selector names.  Masks holding exactly
- * one of LFASM_CROSS0 / LFASM_CROSS1 always use the quadrature kernels.
- * Off: every element takes the quadrature kernels.  Returns the number of
- * elements on the tensor path (0 when off). */
-int lfasm_set_tensor_path(lfasm_handle *h, int enable);
+/* Element-matrix path selection (no reference counterpart).  Elements
+ * whose four vertices lie in [z_min, z_max] of their problem -- Lambda = I
+ * at every quadrature point, so every coefficient is constant over the
+ * element -- can take the reference-tensor path: Ae = sum of 21
+ * precomputed reference-element tensors (built from the same rule tables)
+ * times element coefficients from J, eps and k_par.  The other elements
+ * take the quadrature kernels that lfasm_compute_local_kernel's selector
+ * names.  mode 0: quadrature kernels for every element; 1 (default):
+ * reference-tensor kernel writing Ae; 2: the CSR row kernels of
+ * lfasm_scatter_eliminate evaluate those elements' rows directly from the
+ * tensors (their Ae is never written -- it saves that HBM, but the row
+ * kernels then run one LDS-heavy workgroup per CU and are slower than mode
+ * 1 at 1.3M DOF; lfasm_download_local computes Ae on demand), for p <= 3
+ * with the face transform on, else as mode 1.  Modes 1 and 2 give
+ * bit-identical values.  Masks
+ * holding exactly one of LFASM_CROSS0 / LFASM_CROSS1 always use the
+ * quadrature kernels.  Returns the number of elements on the tensor path
+ * (0 in mode 0). */
+int lfasm_set_tensor_path(lfasm_handle *h, int mode);
 
 /* Captured assembly step (no reference counterpart; the reference launches
  * ComputeLocal and ScatterEliminate one kernel at a time,

@@ -69,7 +69,19 @@ struct lfasm_handle {
    int nconst = 0, nquad = 0;
    int *clist = nullptr, *qlist = nullptr;
    double *Rten = nullptr;
-   bool tensor_path = true;
+   int tensor_mode = 1;       // 0 off, 1 materialised Ae (default), 2 fused into the scatter
+   unsigned char *eclass = nullptr;   // 1: constant-coefficient element
+   double *coef = nullptr;            // ne x TEN_NC element coefficients (fused mode)
+   // fused row units per local-index type (edge, face, interior)
+   bool fused_ok = false;
+   UnitHdr *units[3] = {nullptr, nullptr, nullptr};
+   int *rc_enc = nullptr;             // contributor entries of the fused kernels
+   std::vector<unsigned char> eclass_host;
+   int nunit[3] = {0, 0, 0}, rowcap[3] = {0, 0, 0}, fwaves[3] = {0, 0, 0};
+   int local_mode = 0;        // last compute_local: 0 quadrature, 1 tensor + Ae, 2 fused
+   int cus = 0;               // compute units of the device
+   size_t lds_max = 0;        // LDS bytes one workgroup may hold
+   LocalParams last_P{};
    // global
    int64_t nnz = 0;
    int maxrow = 0;
@@ -114,6 +126,12 @@ struct lfasm_handle {
       }                                                                          \
    } while (0)
 
+#define LF_TRY(call)                                                             \
+   do {                                                                          \
+      hipError_t e_ = (call);                                                    \
+      if (e_ != hipSuccess) { return e_; }                                       \
+   } while (0)
+
 namespace {
 
 template <typename T>
@@ -151,7 +169,7 @@ void release_mesh(lfasm_handle *h)
    dfree(h->J); dfree(h->Ji); dfree(h->detJ); dfree(h->v0);
    dfree(h->attr); dfree(h->fo); dfree(h->dof); dfree(h->sign);
    dfree(h->eps_tab); dfree(h->eps_elem); dfree(h->TInv_dev); dfree(h->Ae);
-   dfree(h->clist); dfree(h->qlist); dfree(h->Rten);
+   dfree(h->clist); dfree(h->qlist); dfree(h->Rten); dfree(h->eclass); dfree(h->coef);
    h->nconst = h->nquad = 0;
    dfree(h->pml_tab); dfree(h->pid); dfree(h->eps_off);
    dfree(h->waves); dfree(h->region_slab); dfree(h->region_src); dfree(h->region_deps);
@@ -169,6 +187,13 @@ void release_global(lfasm_handle *h)
 {
    drop_graph(h);
    dfree(h->Ip); dfree(h->Jc); dfree(h->rc_ptr); dfree(h->rc_list);
+   for (int t = 0; t < 3; ++t)
+   {
+      dfree(h->units[t]);
+      h->nunit[t] = h->rowcap[t] = h->fwaves[t] = 0;
+   }
+   dfree(h->rc_enc);
+   h->fused_ok = false;
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
    dfree(h->rhs);
@@ -336,8 +361,174 @@ hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel)
    return hipErrorInvalidValue;
 }
 
+// Geometry of the LDS slice of local-index type `t` (0 edge, 1 face, 2 interior).
+void slice_geometry(int nd, int p, int t, int &r0, int &nR)
+{
+   const int ne_ = 6 * p, nf = 4 * p * (p - 1);
+   r0 = t == 0 ? 0 : t == 1 ? ne_ : ne_ + nf;
+   nR = t == 0 ? ne_ : t == 1 ? nf : nd - ne_ - nf;
+}
+
+constexpr size_t LDS_MAX = 163840;   // LDS bytes per CU on gfx950
+
+size_t slice_bytes(int nd, int nR)
+{
+   const size_t S = (size_t)nR * (nd - nR) + (size_t)nR * (nR + 1) / 2;
+   return (size_t)NTENS * S * sizeof(double);
+}
+
+// Row units of the fused scatter: every CSR row is typed by the local index
+// its contributors hold (the same type in every element); face rows whose
+// partner row (local a+1 in every contributing element, same elements in
+// the same order) exists form one unit.  A pattern that does not fit (mixed
+// types, more than 64 contributors in a row, rows too long for LDS) keeps
+// the fused path off; the materialised path still works.
+int build_fused_units(lfasm_handle *h, const int *I, const std::vector<int> &dof,
+                      const std::vector<int> &rc_ptr, const std::vector<int> &rc_list,
+                      const std::vector<double> &sign, const std::vector<unsigned char> &ess)
+{
+   const int n = h->height, nd = h->nd, p = h->p;
+   if (nd > 64 || (int64_t)h->ne * nd > CE_MASK) { return LFASM_OK; }
+   std::vector<int> type((size_t)n, 0);
+   int r0s[3], nRs[3];
+   for (int t = 0; t < 3; ++t) { slice_geometry(nd, p, t, r0s[t], nRs[t]); }
+   auto type_of = [&](int a) { return a < r0s[1] ? 0 : a < r0s[2] ? 1 : 2; };
+   for (int r = 0; r < n; ++r)
+   {
+      const int c0 = rc_ptr[(size_t)r], c1 = rc_ptr[(size_t)r + 1];
+      if (c1 - c0 > 64) { return LFASM_OK; }
+      const int t = c0 < c1 ? type_of(rc_list[(size_t)c0] % nd) : 0;
+      for (int c = c0; c < c1; ++c)
+      {
+         if (type_of(rc_list[(size_t)c] % nd) != t) { return LFASM_OK; }
+      }
+      type[(size_t)r] = t;
+   }
+   std::vector<UnitHdr> units[3];
+   std::vector<char> taken((size_t)n, 0);
+   int rowcap[3] = {0, 0, 0};
+   for (int r = 0; r < n; ++r)
+   {
+      const int t = type[(size_t)r];
+      rowcap[t] = std::max(rowcap[t], I[r + 1] - I[r]);
+      if (taken[(size_t)r]) { continue; }
+      taken[(size_t)r] = 1;
+      int partner = -1;
+      const int c0 = rc_ptr[(size_t)r], c1 = rc_ptr[(size_t)r + 1];
+      if (t == 1 && c0 < c1)
+      {
+         const int ea0 = rc_list[(size_t)c0];
+         const int a0 = ea0 % nd;
+         if (((a0 - r0s[1]) & 1) == 0)
+         {
+            const int j = dof[(size_t)ea0 + 1];
+            bool ok = j != r && !taken[(size_t)j] &&
+                      rc_ptr[(size_t)j + 1] - rc_ptr[(size_t)j] == c1 - c0;
+            for (int c = c0; ok && c < c1; ++c)
+            {
+               const int ea = rc_list[(size_t)c];
+               ok = ((ea % nd - r0s[1]) & 1) == 0 &&
+                    rc_list[(size_t)(rc_ptr[(size_t)j] + (c - c0))] == ea + 1;
+            }
+            if (ok) { partner = j; }
+         }
+      }
+      if (partner >= 0) { taken[(size_t)partner] = 1; }
+      UnitHdr u;
+      u.i0 = r;
+      u.i1 = partner;
+      u.base0 = I[r];
+      u.len0 = I[r + 1] - I[r];
+      u.base1 = partner >= 0 ? I[partner] : 0;
+      u.len1 = partner >= 0 ? I[partner + 1] - I[partner] : 0;
+      u.c0 = c0;
+      u.nc_flags = ((c1 - c0) << 8) | (ess[(size_t)r] ? 1 : 0) |
+                   (partner >= 0 && ess[(size_t)partner] ? 2 : 0);
+      units[t].push_back(u);
+   }
+   // contributor entries: e*nd + a, PML bit, signs of rows a and a+1
+   std::vector<int> enc(rc_list.size() + 64, 0);   // 64 pad entries: unguarded list loads
+   for (size_t c = 0; c < rc_list.size(); ++c)
+   {
+      const int ea = rc_list[c];
+      const int a = ea % nd;
+      unsigned v = (unsigned)ea;
+      if (!h->eclass_host[(size_t)(ea / nd)]) { v |= 1u << 31; }
+      if (sign[(size_t)ea] < 0.0) { v |= 1u << 30; }
+      if (a + 1 < nd && sign[(size_t)ea + 1] < 0.0) { v |= 1u << 29; }
+      enc[c] = (int)v;
+   }
+   // launch shape: as many waves as the LDS left by the tensor slice holds
+   int waves[3];
+   for (int t = 0; t < 3; ++t)
+   {
+      waves[t] = 0;
+      if (units[t].empty()) { continue; }
+      const size_t rb = slice_bytes(nd, nRs[t]);
+      const size_t per_wave = (size_t)(t == 1 ? 2 : 1) * std::max(rowcap[t], 1) * sizeof(double2);
+      if (rb + 4 * per_wave > h->lds_max) { return LFASM_OK; }
+      waves[t] = (int)std::min<size_t>(FUSED_MAX_WAVES, (h->lds_max - rb) / per_wave);
+   }
+   for (int t = 0; t < 3; ++t)
+   {
+      LF_HIP(upload(h, h->units[t], units[t].data(), units[t].size()));
+      h->nunit[t] = (int)units[t].size();
+      h->rowcap[t] = std::max(rowcap[t], 1);
+      h->fwaves[t] = waves[t];
+   }
+   LF_HIP(upload(h, h->rc_enc, enc.data(), enc.size()));
+   h->fused_ok = true;
+   return LFASM_OK;
+}
+
+// Element coefficients of the constant-coefficient elements (fused mode).
+hipError_t launch_tensor_coef(lfasm_handle *h, const TensorParams &T)
+{
+   hipLaunchKernelGGL(k_tensor_coef, dim3((unsigned)((T.nlist + 255) / 256)), dim3(256), 0,
+                      h->stream, T, h->coef);
+   return hipGetLastError();
+}
+
+// The three fused row launches (edge, face, interior rows).
+hipError_t launch_fused_rows(lfasm_handle *h, int eliminate)
+{
+   for (int t = 0; t < 3; ++t)
+   {
+      if (h->nunit[t] == 0) { continue; }
+      FusedParams F;
+      F.nd = h->nd;
+      F.p = h->p;
+      F.npair = (h->nd + 1) / 2;
+      slice_geometry(h->nd, h->p, t, F.r0, F.nR);
+      F.no = h->nd - F.nR;
+      F.S = F.nR * F.no + F.nR * (F.nR + 1) / 2;
+      F.face = t == 1 ? 1 : 0;
+      F.nunit = h->nunit[t];
+      F.hdr = h->units[t];
+      F.rowcap = h->rowcap[t];
+      F.Jc = h->Jc;
+      F.eliminate = eliminate ? 1 : 0;
+      F.slot = h->slot;
+      F.eflag = h->eflag;
+      std::memcpy(F.TInv, h->TInv, sizeof(F.TInv));
+      F.Ae = h->Ae;
+      F.R = h->Rten;
+      F.vals = h->vals;
+      const int nw = h->fwaves[t];
+      const size_t shmem = slice_bytes(h->nd, F.nR) +
+                           (size_t)nw * (F.face ? 2 : 1) * F.rowcap * sizeof(double2);
+      LF_TRY(hipFuncSetAttribute((const void *)k_fused_rows,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+      const int grid = std::max(1, std::min(h->cus, (F.nunit + nw - 1) / nw));
+      hipLaunchKernelGGL(k_fused_rows, dim3(grid), dim3(64 * nw), shmem, h->stream, F,
+                         (const double *)h->coef, (const int *)h->rc_enc);
+      LF_TRY(hipGetLastError());
+   }
+   return hipSuccess;
+}
+
 // Reference-tensor kernel over the handle's constant-coefficient elements.
-hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
+TensorParams tensor_params(lfasm_handle *h, const LocalParams &P)
 {
    TensorParams T;
    T.nd = h->nd;
@@ -364,6 +555,12 @@ hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
    T.use_cross = (P.mflags >> 1) & 1;
    T.apply_te = P.apply_te;
    T.Ae = P.Ae;
+   return T;
+}
+
+hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
+{
+   const TensorParams T = tensor_params(h, P);
    const int epar = T.nb > 128 ? 1 : 256 / T.nb;
    const dim3 grid((unsigned)((h->nconst + TEN_EPC - 1) / TEN_EPC),
                    epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
@@ -379,11 +576,6 @@ hipError_t ev_begin(lfasm_handle *h, int t)
 
 hipError_t ev_end(lfasm_handle *h, int t) { return hipEventRecord(h->ev[2 * t + 1], h->stream); }
 
-#define LF_TRY(call)                                                             \
-   do {                                                                          \
-      hipError_t e_ = (call);                                                    \
-      if (e_ != hipSuccess) { return e_; }                                       \
-   } while (0)
 
 // Element matrices of the whole mesh, enqueued on h->stream without a
 // host wait: with the tensor path, the reference-tensor kernel over the
@@ -396,7 +588,13 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
    // the tensor kernel sums CROSS0 and CROSS1 as one antisymmetric term, so
    // a mask holding only one of them goes through the quadrature kernels
    const bool one_cross = ((P.mflags >> 1) & 1) != ((P.mflags >> 2) & 1);
-   const bool split = h->tensor_path && !one_cross && h->nconst > 0;
+   const bool split = h->tensor_mode > 0 && !one_cross && h->nconst > 0;
+   // fused: element coefficients only; the row kernels evaluate the tensor
+   // rows (needs the row units of build_global and the face transform on)
+   const bool fused = split && h->tensor_mode == 2 && h->fused_ok && h->global_built &&
+                      (P.apply_te || h->p < 2);
+   h->last_P = P;
+   h->local_mode = fused ? 2 : split ? 1 : 0;
    if (timed)
    {
       h->ev_used[T_LOCAL_TENSOR] = h->ev_used[T_LOCAL_QUAD] = false;
@@ -411,7 +609,8 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
    else if (split)
    {
       if (timed) { LF_TRY(ev_begin(h, T_LOCAL_TENSOR)); }
-      LF_TRY(launch_tensor(h, P));
+      if (fused) { LF_TRY(launch_tensor_coef(h, tensor_params(h, P))); }
+      else { LF_TRY(launch_tensor(h, P)); }
       if (timed) { LF_TRY(ev_end(h, T_LOCAL_TENSOR)); }
       if (h->nquad > 0)
       {
@@ -435,6 +634,13 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
 // Scatter + elimination of the element matrices into the CSR values.
 hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
 {
+   if (h->local_mode == 2)
+   {
+      if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
+      LF_TRY(launch_fused_rows(h, eliminate));
+      if (timed) { LF_TRY(ev_end(h, T_SCATTER)); }
+      return hipSuccess;
+   }
    ScatterParams S;
    S.n = h->height;
    S.nd = h->nd;
@@ -485,6 +691,12 @@ int lfasm_create(int device, lfasm_handle **out)
    *out = h;
    LF_HIP(hipSetDevice(device));
    LF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+   LF_HIP(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, device));
+   {
+      int lds = 0;
+      LF_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+      h->lds_max = std::min<size_t>((size_t)lds, LDS_MAX);
+   }
 
    for (int i = 0; i < 2 * T_N; ++i) { LF_HIP(hipEventCreate(&h->ev[i])); }
    return LFASM_OK;
@@ -736,6 +948,11 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
       h->nquad = (int)ql.size();
       LF_HIP(upload(h, h->clist, cl.data(), cl.size()));
       LF_HIP(upload(h, h->qlist, ql.data(), ql.size()));
+      std::vector<unsigned char> ec(ne, 0);
+      for (int e : cl) { ec[(size_t)e] = 1; }
+      LF_HIP(upload(h, h->eclass, ec.data(), ec.size()));
+      h->eclass_host = ec;
+      if (m0.p <= 3) { LF_HIP(hipMalloc((void **)&h->coef, ne * TEN_NC * sizeof(double))); }
       const int npair = (m0.nd + 1) / 2;
       const size_t nR = (size_t)npair * npair * NTENS * 4;
       LF_HIP(hipMalloc((void **)&h->Rten, nR * sizeof(double)));
@@ -848,6 +1065,11 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
       release_global(h);
       return h->fail(LFASM_E_PATTERN, "an element (row, col) pair is missing from the CSR pattern");
    }
+   if (h->coef)
+   {
+      const int rc = build_fused_units(h, I, dof, rc_ptr, rc_list, sign, ess);
+      if (rc != LFASM_OK) { release_global(h); return rc; }
+   }
    h->global_built = true;
    return LFASM_OK;
 }
@@ -880,12 +1102,13 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel)
    return LFASM_OK;
 }
 
-int lfasm_set_tensor_path(lfasm_handle *h, int enable)
+int lfasm_set_tensor_path(lfasm_handle *h, int mode)
 {
    if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
-   h->tensor_path = enable != 0;
+   if (mode < 0 || mode > 2) { return h->fail(LFASM_E_ARG, "tensor path mode %d outside 0..2", mode); }
+   h->tensor_mode = mode;
    drop_graph(h);
-   return h->tensor_path ? h->nconst : 0;
+   return mode ? h->nconst : 0;
 }
 
 int lfasm_step_capture(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te, int eliminate)
@@ -986,6 +1209,11 @@ int lfasm_download_local(lfasm_handle *h, double *Ae)
    std::printf("asm_gpu_test_download (test-only D2H)\n");
    std::fflush(stdout);
    LF_HIP(hipSetDevice(h->device));
+   if (h->local_valid && h->local_mode == 2)
+   {
+      // the fused path never wrote the constant-coefficient elements' Ae
+      LF_HIP(launch_tensor(h, h->last_P));
+   }
    LF_HIP(hipMemcpyAsync(Ae, h->Ae, (size_t)h->ne * h->nd * h->nd * sizeof(double2),
                          hipMemcpyDeviceToHost, h->stream));
    LF_HIP(hipStreamSynchronize(h->stream));

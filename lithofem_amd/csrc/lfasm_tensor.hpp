@@ -51,7 +51,11 @@ __global__ void k_ref_tensor(int nd, int npair, RuleDev rc, RuleDev rm, double *
       const int t = (idx >> 2) % NTENS;
       const int blk = (idx >> 2) / NTENS;
       const int rp = blk / npair, cp = blk - rp * npair;
-      const int k = 2 * rp + (s >> 1), l = 2 * cp + (s & 1);
+      const int k0 = 2 * rp + (s >> 1), l0 = 2 * cp + (s & 1);
+      // computed for k <= l and mirrored, so R_cc, R_m are exactly
+      // symmetric and Q exactly antisymmetric (the fused row kernels store
+      // one triangle)
+      const int k = min(k0, l0), l = max(k0, l0);
       double acc = 0.0;
       if (k < nd && l < nd)
       {
@@ -77,6 +81,7 @@ __global__ void k_ref_tensor(int nd, int npair, RuleDev rc, RuleDev rm, double *
                const double *C = rm.cs + (size_t)q * nd * 3;
                acc += rm.w[q] * (V[k * 3 + a] * C[l * 3 + b] - V[l * 3 + a] * C[k * 3 + b]);
             }
+            if (k0 > l0) { acc = -acc; }
          }
       }
       R[idx] = acc;
@@ -152,6 +157,17 @@ __device__ inline void tensor_coef(const TensorParams &P, int e, double *c)
    c[27] = 0.0;
 }
 
+// Dual face transform of one DOF pair (lo, hi) with the 2x2 block Ti, with
+// explicit FMAs so every kernel that applies it rounds the same way.
+__device__ __forceinline__ double pair_lo(const double *Ti, double lo, double hi)
+{
+   return __builtin_fma(Ti[1], hi, Ti[0] * lo);
+}
+__device__ __forceinline__ double pair_hi(const double *Ti, double lo, double hi)
+{
+   return __builtin_fma(Ti[3], hi, Ti[2] * lo);
+}
+
 // Face of a DOF pair starting at even index d0 (-1 if not a face pair).
 __device__ __forceinline__ int pair_face(int d0, int p)
 {
@@ -210,12 +226,12 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
 #pragma unroll
          for (int t = 0; t < 6; ++t)
          {
-            x += c[t] * r[t][s];
-            x += c[6 + 2 * t] * r[6 + t][s];
-            y += c[7 + 2 * t] * r[6 + t][s];
+            x = __builtin_fma(c[t], r[t][s], x);
+            x = __builtin_fma(c[6 + 2 * t], r[6 + t][s], x);
+            y = __builtin_fma(c[7 + 2 * t], r[6 + t][s], y);
          }
 #pragma unroll
-         for (int t = 0; t < 9; ++t) { y += c[18 + t] * r[12 + t][s]; }
+         for (int t = 0; t < 9; ++t) { y = __builtin_fma(c[18 + t], r[12 + t][s], y); }
          re[s] = x;
          im[s] = y;
       }
@@ -228,10 +244,10 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
          {
             const double ar = re[2 * row], ai = im[2 * row];
             const double br = re[2 * row + 1], bi = im[2 * row + 1];
-            re[2 * row] = Ti[0] * ar + Ti[1] * br;
-            im[2 * row] = Ti[0] * ai + Ti[1] * bi;
-            re[2 * row + 1] = Ti[2] * ar + Ti[3] * br;
-            im[2 * row + 1] = Ti[2] * ai + Ti[3] * bi;
+            re[2 * row] = pair_lo(Ti, ar, br);
+            im[2 * row] = pair_lo(Ti, ai, bi);
+            re[2 * row + 1] = pair_hi(Ti, ar, br);
+            im[2 * row + 1] = pair_hi(Ti, ai, bi);
          }
       }
       if (fr >= 0)
@@ -242,10 +258,10 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
          {
             const double ar = re[col], ai = im[col];
             const double br = re[2 + col], bi = im[2 + col];
-            re[col] = Ti[0] * ar + Ti[1] * br;
-            im[col] = Ti[0] * ai + Ti[1] * bi;
-            re[2 + col] = Ti[2] * ar + Ti[3] * br;
-            im[2 + col] = Ti[2] * ai + Ti[3] * bi;
+            re[col] = pair_lo(Ti, ar, br);
+            im[col] = pair_lo(Ti, ai, bi);
+            re[2 + col] = pair_hi(Ti, ar, br);
+            im[2 + col] = pair_hi(Ti, ai, bi);
          }
       }
       double2 *A = P.Ae + (size_t)sel[i] * nd * nd;
@@ -255,6 +271,356 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
          const int k = k0 + (s >> 1), l = l0 + (s & 1);
          if (k < nd && l < nd) { A[(size_t)k * nd + l] = make_double2(re[s], im[s]); }
       }
+   }
+}
+
+}  // namespace lfasm
+
+namespace lfasm {
+
+// ---- reference-tensor rows fused into the row scatter ----------------------
+//
+// For constant-coefficient elements the element matrix is never written to
+// HBM: the CSR row kernel evaluates each contributor row (e, a) from the 27
+// element coefficients and the reference tensors held in LDS.  Rows are
+// grouped by the type of their local index -- edge [0, 6p), face
+// [6p, 6p + 4p(p-1)), interior -- because every element sees a global DOF
+// with a local index of the same type; one launch per type keeps only the
+// tensor rows of that type in LDS: R_t[a][b] for a in the type's range, all
+// b, with the in-range diagonal block stored as a triangle (R_cc, R_m
+// symmetric, Q antisymmetric).  Face rows come in pairs (the two DOFs of a
+// face point, local a and a+1 in every element) that the dual face
+// transform mixes: a unit of work is such a pair of CSR rows, so each
+// tensor row is read once.  PML elements' matrices come from Ae as in
+// k_scatter_rows.  Contributors are summed in ascending element order, as
+// in k_scatter_rows, and each entry is formed with the same FMAs as in
+// k_local_tensor, so both paths give bit-identical values.
+//
+// One workgroup per CU holds the slice, so each wave keeps many loads in
+// flight itself: a unit's header is one 32-byte load, its contributor list
+// one vector load (class and row signs packed into each entry), the next
+// unit's header and list are fetched while the current one is processed,
+// and the loads of FG contributors (coefficients as one vector load with a
+// lane per coefficient, broadcast by readlane) are issued before their
+// in-order accumulation.
+
+// Element coefficients of the tensor path, plus the face orientations
+// packed into the spare slot 27 (fo_f in bits 3f..3f+2).
+__global__ void __launch_bounds__(256) k_tensor_coef(TensorParams P, double *__restrict__ coef)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= P.nlist) { return; }
+   const int e = P.elist[i];
+   double c[TEN_NC];
+   tensor_coef(P, e, c);
+   int packed = 0;
+   if (P.p >= 2)
+   {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) { packed |= P.fo[(size_t)e * 4 + f] << (3 * f); }
+   }
+   c[27] = (double)packed;
+#pragma unroll
+   for (int t = 0; t < TEN_NC; ++t) { coef[(size_t)e * TEN_NC + t] = c[t]; }
+}
+
+// One unit of the fused row kernels: CSR row i0 and its face partner i1
+// (-1 if none), their value ranges, and the contributor list of i0
+// (i1's list is i0's with a -> a + 1).  flags: bit0 row i0 essential,
+// bit1 row i1 essential.
+struct alignas(16) UnitHdr {
+   int i0, i1, base0, len0, base1, len1, c0, nc_flags;   // nc in bits 8.., flags in 0..7
+};
+
+// Contributor entry of the fused kernels: e * nd + a in the low 29 bits,
+// bit 31 = PML element (Ae from memory), bit 30 = sign(e, a) < 0,
+// bit 29 = sign(e, a + 1) < 0.
+constexpr int CE_MASK = (1 << 29) - 1;
+
+struct FusedParams {
+   int nd, p, npair;
+   int r0, nR, no, S;          // local rows [r0, r0 + nR); no = nd - nR; S = per-tensor stride
+   int face;                   // 1: face rows (row transform, pair units)
+   int nunit;
+   const UnitHdr *hdr;
+   int rowcap;                 // longest CSR row of this type
+   const int *Jc;
+   int eliminate;
+   const uint16_t *slot;
+   const unsigned char *eflag;
+   double TInv[24];
+   const double2 *Ae;
+   const double *R;            // blocked reference tensors (k_ref_tensor)
+   double2 *vals;
+};
+
+constexpr int FG = 4;          // contributors whose loads are issued together
+constexpr int FUSED_MAX_WAVES = 12;   // launch bound of k_fused_rows (768 threads)
+
+__device__ __forceinline__ int tri_off(int i, int j, int n) { return i * n - (i * (i - 1)) / 2 + (j - i); }
+
+// LDS offset of R_t[r0 + ar][b] within one tensor's slice; `swapped` marks
+// a lower-triangle entry read from its mirror (negated for Q).
+__device__ __forceinline__ int slice_off(const FusedParams &F, int ar, int b, bool &swapped)
+{
+   const int br = b - F.r0;
+   swapped = false;
+   if (br < 0 || br >= F.nR) { return ar * F.no + (br < 0 ? b : b - F.nR); }
+   if (br >= ar) { return F.nR * F.no + tri_off(ar, br, F.nR); }
+   swapped = true;
+   return F.nR * F.no + tri_off(br, ar, F.nR);
+}
+
+__device__ __forceinline__ double ref_entry(const double *R, int npair, int t, int k, int l)
+{
+   const int blk = (k >> 1) * npair + (l >> 1);
+   return R[((size_t)blk * NTENS + t) * 4 + (k & 1) * 2 + (l & 1)];
+}
+
+// value of lane `src` (wave-uniform) of a double held one per lane
+__device__ __forceinline__ double bcast(double v, int src)
+{
+   const long long x = __double_as_longlong(v);
+   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, src);
+   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), src);
+   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Row ar of the element matrix at this lane's column (before any face
+// transform) from coefficients held one per lane in cl; same FMA order as
+// k_local_tensor.
+__device__ __forceinline__ void tensor_row(const double *Rs, int S, int off, double qs,
+                                           double cl, double &x, double &y)
+{
+   x = 0.0;
+   y = 0.0;
+#pragma unroll
+   for (int t = 0; t < 6; ++t)
+   {
+      const double rc = Rs[t * S + off];
+      const double rm = Rs[(6 + t) * S + off];
+      x = __builtin_fma(bcast(cl, t), rc, x);
+      x = __builtin_fma(bcast(cl, 6 + 2 * t), rm, x);
+      y = __builtin_fma(bcast(cl, 7 + 2 * t), rm, y);
+   }
+#pragma unroll
+   for (int t = 0; t < 9; ++t)
+   {
+      y = __builtin_fma(bcast(cl, 18 + t), qs * Rs[(12 + t) * S + off], y);
+   }
+}
+
+__device__ __forceinline__ UnitHdr load_hdr(const UnitHdr *h, int u)
+{
+   const int4 *p = reinterpret_cast<const int4 *>(h + u);
+   const int4 a = p[0], b = p[1];
+   return UnitHdr{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+// coef and rc_enc are separate restrict arguments so that their
+// wave-uniform loads may go through the scalar cache.
+__global__ void __launch_bounds__(768) k_fused_rows(FusedParams F,
+                                                     const double *__restrict__ coef,
+                                                     const int *__restrict__ rc_enc)
+{
+   extern __shared__ double smem[];
+   double *Rs = smem;
+   const int nthr = blockDim.x;
+   {
+      const int per_t = F.nR * F.nd;
+      for (int idx = threadIdx.x; idx < NTENS * per_t; idx += nthr)
+      {
+         const int t = idx / per_t;
+         const int rem = idx - t * per_t;
+         const int ar = rem / F.nd, b = rem - ar * F.nd;
+         bool sw;
+         const int off = slice_off(F, ar, b, sw);
+         if (!sw) { Rs[(size_t)t * F.S + off] = ref_entry(F.R, F.npair, t, F.r0 + ar, b); }
+      }
+   }
+   __syncthreads();
+   const int nw = nthr >> 6;
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int nbuf = F.face ? 2 : 1;
+   double2 *buf0 = reinterpret_cast<double2 *>(Rs + (size_t)NTENS * F.S) +
+                   (size_t)wave * nbuf * F.rowcap;
+   double2 *buf1 = buf0 + F.rowcap;
+   const int nd = F.nd, p = F.p;
+   const int foff = 6 * p, ntd = p * (p - 1);
+   const unsigned char ess_bit = F.eliminate ? 2 : 0;
+   const int b = lane;                         // nd <= 64 on this path
+   const bool lane_live = b < nd;
+   const int bb = lane_live ? b : 0;
+   const int bface = (b >= foff && b < foff + 4 * ntd) ? (b - foff) / ntd : -1;
+   const bool bodd = ((b - foff) & 1) != 0;
+   const int stride = gridDim.x * nw;
+
+   int u = blockIdx.x * nw + wave;
+   if (u >= F.nunit) { return; }
+   UnitHdr H = load_hdr(F.hdr, u);
+   // rc_enc carries 64 trailing pad entries: the list load needs no guard
+   int encL = rc_enc[H.c0 + lane];
+   for (;;)
+   {
+      const int un = u + stride;
+      const bool more = un < F.nunit;
+      const UnitHdr Hn = load_hdr(F.hdr, more ? un : u);
+      const int nc = H.nc_flags >> 8;
+      const bool pair = H.i1 >= 0;
+      const bool ess0 = F.eliminate && (H.nc_flags & 1);
+      const bool ess1 = F.eliminate && (H.nc_flags & 2);
+      for (int s = lane; s < H.len0; s += 64) { buf0[s] = make_double2(0.0, 0.0); }
+      for (int s = lane; s < H.len1; s += 64) { buf1[s] = make_double2(0.0, 0.0); }
+      wave_sync_lds();
+      const bool work = !ess0 || (pair && !ess1);
+      int encN = 0;
+      bool have_next = false;
+      for (int g0 = 0; work && g0 < nc; g0 += FG)
+      {
+         int enc[FG], sl0[FG], sl1[FG];
+         unsigned char fb[FG];
+         double cl[FG];
+         double2 ap0[FG], ap1[FG];
+#pragma unroll
+         for (int k = 0; k < FG; ++k)
+         {
+            // a past-the-end slot of the group repeats the last contributor
+            const int kk = min(g0 + k, nc - 1);
+            enc[k] = __builtin_amdgcn_readlane(encL, kk);
+            const int ea = enc[k] & CE_MASK;
+            const int e = ea / nd;
+            fb[k] = F.eflag[(size_t)e * nd + bb];
+            sl0[k] = F.slot[(size_t)ea * nd + bb];
+            sl1[k] = pair ? F.slot[(size_t)(ea + 1) * nd + bb] : 0;
+            cl[k] = coef[(size_t)e * TEN_NC + min(lane, TEN_NC - 1)];
+            ap0[k] = make_double2(0.0, 0.0);
+            ap1[k] = make_double2(0.0, 0.0);
+            if (enc[k] < 0)
+            {
+               const dbl2v x = __builtin_nontemporal_load(
+                  reinterpret_cast<const dbl2v *>(F.Ae) + (size_t)ea * nd + bb);
+               ap0[k] = make_double2(x.x, x.y);
+               if (pair)
+               {
+                  const dbl2v y = __builtin_nontemporal_load(
+                     reinterpret_cast<const dbl2v *>(F.Ae) + (size_t)(ea + 1) * nd + bb);
+                  ap1[k] = make_double2(y.x, y.y);
+               }
+            }
+         }
+         if (!have_next)
+         {
+            encN = rc_enc[Hn.c0 + lane];
+            have_next = true;
+         }
+#pragma unroll
+         for (int k = 0; k < FG; ++k)
+         {
+            if (g0 + k >= nc) { break; }
+            const int ea = enc[k] & CE_MASK;
+            const int e = ea / nd, a = ea - e * nd;
+            double2 v0 = ap0[k], v1 = ap1[k];
+            if (enc[k] >= 0)
+            {
+               const int ar = a - F.r0;
+               const int a_lo = F.face ? F.r0 + (ar & ~1) : a;
+               double xr, xi, yr = 0.0, yi = 0.0;
+               bool sw;
+               int off = slice_off(F, a_lo - F.r0, bb, sw);
+               tensor_row(Rs, F.S, off, sw ? -1.0 : 1.0, cl[k], xr, xi);
+               if (F.face)
+               {
+                  off = slice_off(F, a_lo + 1 - F.r0, bb, sw);
+                  tensor_row(Rs, F.S, off, sw ? -1.0 : 1.0, cl[k], yr, yi);
+               }
+               if (!lane_live) { xr = xi = yr = yi = 0.0; }
+               const int fop = (int)bcast(cl[k], 27);
+               // dual face transform: columns (lane pairs), then rows
+               {
+                  const double pxr = __shfl_xor(xr, 1), pxi = __shfl_xor(xi, 1);
+                  const double pyr = __shfl_xor(yr, 1), pyi = __shfl_xor(yi, 1);
+                  if (bface >= 0)
+                  {
+                     const double *Ti = F.TInv + 4 * ((fop >> (3 * bface)) & 7);
+                     if (!bodd)
+                     {
+                        xr = pair_lo(Ti, xr, pxr);
+                        xi = pair_lo(Ti, xi, pxi);
+                        yr = pair_lo(Ti, yr, pyr);
+                        yi = pair_lo(Ti, yi, pyi);
+                     }
+                     else
+                     {
+                        xr = pair_hi(Ti, pxr, xr);
+                        xi = pair_hi(Ti, pxi, xi);
+                        yr = pair_hi(Ti, pyr, yr);
+                        yi = pair_hi(Ti, pyi, yi);
+                     }
+                  }
+               }
+               if (F.face)
+               {
+                  const int f = (a_lo - foff) / ntd;
+                  const double *Ti = F.TInv + 4 * ((fop >> (3 * f)) & 7);
+                  const double lr = pair_lo(Ti, xr, yr), li = pair_lo(Ti, xi, yi);
+                  const double hr = pair_hi(Ti, xr, yr), hi = pair_hi(Ti, xi, yi);
+                  if (pair || a == a_lo)
+                  {
+                     v0 = make_double2(lr, li);
+                     v1 = make_double2(hr, hi);
+                  }
+                  else
+                  {
+                     v0 = make_double2(hr, hi);
+                  }
+               }
+               else
+               {
+                  v0 = make_double2(xr, xi);
+               }
+            }
+            const bool use = lane_live && (fb[k] & ess_bit) == 0;
+            if (use && !ess0)
+            {
+               double2 t = buf0[sl0[k]];
+               if (((enc[k] >> 30) ^ fb[k]) & 1) { t.x -= v0.x; t.y -= v0.y; }
+               else { t.x += v0.x; t.y += v0.y; }
+               buf0[sl0[k]] = t;
+            }
+            if (pair && use && !ess1)
+            {
+               double2 t = buf1[sl1[k]];
+               if (((enc[k] >> 29) ^ fb[k]) & 1) { t.x -= v1.x; t.y -= v1.y; }
+               else { t.x += v1.x; t.y += v1.y; }
+               buf1[sl1[k]] = t;
+            }
+            wave_sync_lds();
+         }
+      }
+      if (!have_next) { encN = rc_enc[Hn.c0 + lane]; }
+      if (ess0)
+      {
+         for (int s = lane; s < H.len0; s += 64)
+         {
+            if (F.Jc[H.base0 + s] == H.i0) { buf0[s] = make_double2(1.0, 0.0); }
+         }
+      }
+      if (ess1)
+      {
+         for (int s = lane; s < H.len1; s += 64)
+         {
+            if (F.Jc[H.base1 + s] == H.i1) { buf1[s] = make_double2(1.0, 0.0); }
+         }
+      }
+      wave_sync_lds();
+      for (int s = lane; s < H.len0; s += 64) { F.vals[H.base0 + s] = buf0[s]; }
+      for (int s = lane; s < H.len1; s += 64) { F.vals[H.base1 + s] = buf1[s]; }
+      wave_sync_lds();
+      if (!more) { break; }
+      u = un;
+      H = Hn;
+      encL = encN;
    }
 }
 

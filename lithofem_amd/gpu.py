@@ -362,10 +362,15 @@ class AsmGpuGlobal:
     def step_launch(self, sync=True):
         self._check(lib().lfasm_step_launch(self._h, 1 if sync else 0))
 
-    def set_tensor_path(self, enable: bool) -> int:
-        """Reference-tensor kernel for constant-coefficient elements (on by
-        default); returns the number of elements it covers."""
-        rc = lib().lfasm_set_tensor_path(self._h, 1 if enable else 0)
+    TENSOR_OFF, TENSOR_AE, TENSOR_FUSED = 0, 1, 2
+
+    def set_tensor_path(self, mode) -> int:
+        """Element path of constant-coefficient elements: 0 (or False)
+        quadrature kernels, 1 (or True, the default) reference-tensor kernel
+        writing Ae, 2 tensor rows evaluated inside the CSR row kernels.
+        Returns the number of elements on the tensor path."""
+        mode = 1 if mode is True else 0 if mode is False else int(mode)
+        rc = lib().lfasm_set_tensor_path(self._h, mode)
         if rc < 0:
             raise AsmGpuError(rc, self.last_error())
         return rc

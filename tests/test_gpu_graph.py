@@ -19,14 +19,16 @@ from lithofem_amd import gpu
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("mode", [1, 2], ids=["tensor", "fused"])
 @pytest.mark.parametrize("p", [2, 3])
-def test_captured_step_replays_equal_eager(p):
+def test_captured_step_replays_equal_eager(p, mode):
     c, ad, I, J = case("grating", p, 6.0, "small")
-    print(f"graph p={p} ne={ad.ne} nnz={len(J)}", flush=True)
+    print(f"graph p={p} tensor mode {mode} ne={ad.ne} nnz={len(J)}", flush=True)
     g = gpu.AsmGpuGlobal(0)
     try:
         g.init(ad, c.problem)
         g.build_global(I, J, ad.ess)
+        g.set_tensor_path(mode)
         with pytest.raises(gpu.AsmGpuError):
             g.step_launch()                      # nothing captured yet
         g.compute_local()

@@ -157,6 +157,54 @@ def test_u5_u6_global(name, p, theta):
         g.close()
 
 
+@pytest.mark.parametrize("renumber", [False, True], ids=["native", "permuted"])
+@pytest.mark.parametrize("p", [1, 2, 3], ids=["p1", "p2", "p3"])
+def test_tensor_modes_global(p, renumber):
+    """The three element paths give the same eliminated system.  With the
+    global DOFs renumbered at random the face-row pairs of the fused path
+    break up (single-row units) -- the path a caller's own numbering can
+    take -- and the result still matches the oracle."""
+    import dataclasses
+    c, ad, I, J = case("hole", p, 6.0, "small")
+    if renumber:
+        rng = np.random.default_rng(17 + p)
+        perm = rng.permutation(ad.height).astype(np.int32)
+        ad = dataclasses.replace(ad, dof=np.ascontiguousarray(perm[ad.dof]),
+                                 ess=np.sort(perm[ad.ess]).astype(np.int32))
+        from lithofem_amd.fespace import csr_symbolic
+        I, J = csr_symbolic(ad.dof, ad.height)
+    orc = O.Oracle(ad, c.problem, I, J)
+    rre, rim = orc.assemble()
+    is_ess = np.zeros(ad.height, np.uint8)
+    is_ess[ad.ess] = 1
+    xre, xim = O.eliminate(I, J, is_ess, rre, rim)
+    g = _ctx(c, ad, I, J)
+    try:
+        vals = {}
+        for mode in (2, 1, 0):
+            n = g.set_tensor_path(mode)
+            assert (n > 0) == (mode > 0)
+            g.compute_local()
+            g.scatter_eliminate(True)
+            vre, vim = g.download_values()
+            vals[mode] = (vre, vim)
+            err = maxnorm_err(vre + 1j * vim, xre + 1j * xim)
+            print(f"  mode {mode}: vs oracle {err:.2e}", flush=True)
+            assert err < TOL, (mode, err)
+        same = np.array_equal(vals[2][0], vals[1][0]) and np.array_equal(vals[2][1], vals[1][1])
+        print(f"  fused == materialised bitwise: {same}", flush=True)
+        # the on-demand element matrices of the fused path
+        g.set_tensor_path(2)
+        g.compute_local()
+        Af = g.download_local()
+        g.set_tensor_path(1)
+        g.compute_local()
+        Am = g.download_local()
+        assert np.array_equal(Af, Am)
+    finally:
+        g.close()
+
+
 def test_force_fail_env(monkeypatch):
     c, ad, I, J = case("grating", 2, 6.0, "tiny")
     monkeypatch.setenv("LITHOFEM_ASM_FORCE_FAIL", "1")
