@@ -79,7 +79,7 @@ def main() -> int:
                     help="constant-coefficient elements: reference-tensor kernel (on), "
                          "quadrature kernels (off), tensor rows inside the scatter (fused)")
     ap.add_argument("--mode", default="assemble",
-                    choices=["assemble", "multimesh", "variants", "rhs", "adjoint"],
+                    choices=["assemble", "multimesh", "variants", "rhs", "adjoint", "large"],
                     help="assemble: the headline step; others: batched / adjoint workloads")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -210,7 +210,8 @@ int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
 
 /* Device-event timings (milliseconds) of the last calls, in this order:
  * [0] compute_local kernels, [1] scatter_eliminate kernels,
- * [2] build_global device part, [3] last batched call, [4] last adjoint call,
+ * [2] build_global device part, [3] last batched call (variants_assemble or
+ * assemble_rhs kernels), [4] last adjoint call,
  * [5] reference-tensor part and [6] quadrature part of the last
  * compute_local (0 when that part did not run), [7] last lfasm_step_launch. */
 int lfasm_timings(const lfasm_handle *h, double *ms, int n);

@@ -1479,6 +1479,7 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
    {
       e = hipMalloc((void **)&bloc, src.size() * nrhs * nd * sizeof(double2));
    }
+   if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
    if (e == hipSuccess && !src.empty())
    {
       RhsParams R;
@@ -1529,6 +1530,7 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
       }
       e = hipGetLastError();
    }
+   if (e == hipSuccess) { e = ev_end(h, T_BATCH); }
    if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
    if (d_src) { (void)hipFree(d_src); }
    if (d_index) { (void)hipFree(d_index); }
