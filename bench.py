@@ -116,6 +116,17 @@ def main() -> int:
     g.init(ad, pb)
     g.build_global(I, J, ad.ess)
     t_upload = time.perf_counter() - t0
+    # the same pattern built on the device (replaces the host symbolic phase)
+    t0 = time.perf_counter()
+    g.build_global_device(ad.ess)
+    t_dev_build = time.perf_counter() - t0
+    pattern_ms = g.timings()["pattern_ms"]
+    Id, Jd = g.download_pattern()
+    same_pattern = bool(np.array_equal(Id, I) and np.array_equal(Jd, J))
+    del Id, Jd
+    log(f"device pattern: build_global_device {t_dev_build:.3f}s (pattern phase "
+        f"{pattern_ms:.1f} ms) vs host csr_symbolic {t_csr:.3f}s + build_global {t_upload:.3f}s; "
+        f"identical: {same_pattern}")
     nconst = g.set_tensor_path({"on": 1, "off": 0, "fused": 2}[args.tensor])
     nquad = ad.ne - nconst
     log(f"element classes: {nconst} constant-coefficient (reference-tensor kernel), "
@@ -181,7 +192,8 @@ def main() -> int:
     t0 = time.perf_counter()
     g.download_values()
     download_ms = (time.perf_counter() - t0) * 1e3
-    log(f"download of the CSR values (re/im split, pageable host memory): {download_ms:.1f} ms")
+    log(f"download of the CSR values (pinned staging, re/im split on host threads): "
+        f"{download_ms:.1f} ms")
 
     W = canonical_flops(ad.p, ad.nd, ad.rule_cc.nq, ad.rule_mass.nq, pb.has_kpar)
     q_sc = scatter_bytes(ad.ne, ad.nd, ad.height, len(J))
@@ -239,7 +251,11 @@ def main() -> int:
                    "download_values_ms": download_ms,
                    "elements_per_s_incl_download": ad.ne / ((ms_step + download_ms) * 1e-3),
                    "local_min_bytes": local_bytes(ad.ne, ad.nd),
-                   "setup_s": {"extract": t_extract, "csr": t_csr, "upload": t_upload}},
+                   "setup_s": {"extract": t_extract, "csr_host_threads": t_csr,
+                               "init_and_build_global": t_upload,
+                               "build_global_device": t_dev_build,
+                               "device_pattern_phase_ms": pattern_ms,
+                               "device_pattern_identical": same_pattern}},
     }
     if not args.no_cpu:
         cb = cpu_baseline(wl, I, J)

@@ -128,6 +128,17 @@ int lfasm_set_eps_elem(lfasm_handle *h, const double *eps);
 int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t nnz,
                        const unsigned char *is_ess);
 
+/* build_csr_symbolic (solver/asm_data.inc:178-210) + AsmGpuGlobal::BuildGlobal
+ * in one call, with the CSR pattern built on the device from the element dof
+ * table: sorted-column rows of the union of the element dof x dof graphs,
+ * the same pattern as the host routine (and as MFEM's after
+ * SortColumnIndices).  Then as lfasm_build_global.  lfasm_sizes reports nnz;
+ * lfasm_device_csr / lfasm_download_pattern return the pattern. */
+int lfasm_build_global_device(lfasm_handle *h, const unsigned char *is_ess);
+
+/* Host copy of the CSR pattern (I: height+1, Jcol: nnz); either may be NULL. */
+int lfasm_download_pattern(lfasm_handle *h, int *I, int *Jcol);
+
 /* AsmGpu::ComputeLocal (solver/asm_gpu.inc:326-351): element matrices of
  * the CurlCurlIntegrator (mask_cc & LFASM_CURLCURL) on the 2p-2 rule plus
  * the masked mass / mixed / KLK integrators (mask_mass) on the 2p rule,
@@ -213,7 +224,9 @@ int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
  * [2] build_global device part, [3] last batched call (variants_assemble or
  * assemble_rhs kernels), [4] last adjoint call,
  * [5] reference-tensor part and [6] quadrature part of the last
- * compute_local (0 when that part did not run), [7] last lfasm_step_launch. */
+ * compute_local (0 when that part did not run), [7] last lfasm_step_launch,
+ * [8] lfasm_build_global_device's pattern phase (both kernel passes and the
+ * row-pointer scan between them). */
 int lfasm_timings(const lfasm_handle *h, double *ms, int n);
 
 /* ---- eps variants (no reference counterpart) ------------------------------

@@ -8,6 +8,7 @@
 #include "lfasm_peaks.hpp"
 #include "lfasm_batch.hpp"
 #include "lfasm_tensor.hpp"
+#include "lfasm_pattern.hpp"
 #include "../../include/lithofem_asm.h"
 
 #include <hip/hip_runtime.h>
@@ -32,7 +33,7 @@ struct DevRule {
 
 enum {
    T_LOCAL = 0, T_SCATTER = 1, T_BUILD = 2, T_BATCH = 3, T_ADJ = 4,
-   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_STEP = 7, T_N = 8
+   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_STEP = 7, T_PATTERN = 8, T_N = 9
 };
 
 }  // namespace
@@ -45,6 +46,9 @@ struct lfasm_handle {
    hipGraphExec_t graph_exec = nullptr;
    bool graph_dirty = true;
    bool g_have = false;                       // capture parameters set
+   // pinned staging of the host handoff (two chunks, allocated on first use)
+   double2 *pin[2] = {nullptr, nullptr};
+   hipEvent_t pin_ev[2] = {nullptr, nullptr};
    int g_mask_cc = 0, g_mask_mass = 0, g_apply_te = 0, g_elim = 0;
    std::string err;
    // mesh
@@ -714,6 +718,11 @@ void lfasm_destroy(lfasm_handle *h)
       if (h->ev[i]) { (void)hipEventDestroy(h->ev[i]); }
    }
    drop_graph(h);
+   for (int k = 0; k < 2; ++k)
+   {
+      if (h->pin[k]) { (void)hipHostFree(h->pin[k]); }
+      if (h->pin_ev[k]) { (void)hipEventDestroy(h->pin_ev[k]); }
+   }
    if (h->stream) { (void)hipStreamDestroy(h->stream); }
    delete h;
 }
@@ -994,48 +1003,39 @@ int lfasm_set_eps_elem(lfasm_handle *h, const double *eps)
    return LFASM_OK;
 }
 
-int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t nnz,
-                       const unsigned char *is_ess)
+}  // extern "C"
+
+namespace {
+
+// Per-row contributor lists (element, local index) in ascending element
+// order: counting sort of the element dof table.
+void host_contributors(int n, const std::vector<int> &dof, std::vector<int> &rc_ptr,
+                       std::vector<int> &rc_list)
 {
-   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
-   if (!I || !Jcol || nnz <= 0 || nnz > INT32_MAX)
-   {
-      return h->fail(LFASM_E_ARG, "CSR pattern missing or nnz out of int32 range");
-   }
-   const int n = h->height;
-   if (I[0] != 0 || I[n] != nnz) { return h->fail(LFASM_E_ARG, "I[0] != 0 or I[n] != nnz"); }
-   int maxrow = 0;
-   for (int r = 0; r < n; ++r)
-   {
-      const int len = I[r + 1] - I[r];
-      if (len < 0) { return h->fail(LFASM_E_ARG, "I not monotone at row %d", r); }
-      maxrow = std::max(maxrow, len);
-   }
-   if (maxrow > 65535) { return h->fail(LFASM_E_UNSUPPORTED, "row length %d > 65535", maxrow); }
-   for (int64_t k = 0; k < nnz; ++k)
-   {
-      if (Jcol[k] < 0 || Jcol[k] >= n) { return h->fail(LFASM_E_ARG, "Jcol[%lld] out of range", (long long)k); }
-   }
-   LF_HIP(hipSetDevice(h->device));
-   release_global(h);
-   h->nnz = nnz;
-   h->maxrow = maxrow;
-   // per-row contributor lists (element, local index), ascending element
-   // order: counting sort of the element dof table
-   const size_t ne = (size_t)h->ne, nd = (size_t)h->nd;
-   std::vector<int> dof(ne * nd);
-   LF_HIP(hipMemcpy(dof.data(), h->dof, dof.size() * sizeof(int), hipMemcpyDeviceToHost));
-   std::vector<int> rc_ptr((size_t)n + 1, 0), rc_list(ne * nd);
-   for (size_t i = 0; i < ne * nd; ++i) { rc_ptr[(size_t)dof[i] + 1]++; }
+   rc_ptr.assign((size_t)n + 1, 0);
+   rc_list.assign(dof.size(), 0);
+   for (size_t i = 0; i < dof.size(); ++i) { rc_ptr[(size_t)dof[i] + 1]++; }
    for (int r = 0; r < n; ++r) { rc_ptr[(size_t)r + 1] += rc_ptr[r]; }
+   std::vector<int> pos(rc_ptr.begin(), rc_ptr.end() - 1);
+   for (size_t i = 0; i < dof.size(); ++i) { rc_list[(size_t)pos[dof[i]]++] = (int)i; }
+}
+
+// Everything of build_global after the pattern (h->Ip, h->Jc on the device,
+// I on the host) and the contributor lists are in place.
+int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
+                  const std::vector<int> &rc_ptr, const std::vector<int> &rc_list,
+                  const unsigned char *is_ess)
+{
+   const int n = h->height;
+   const size_t ne = (size_t)h->ne, nd = (size_t)h->nd;
+   int maxrow = 0;
+   for (int r = 0; r < n; ++r) { maxrow = std::max(maxrow, I[r + 1] - I[r]); }
+   if (maxrow > 65535)
    {
-      std::vector<int> pos(rc_ptr.begin(), rc_ptr.end() - 1);
-      for (size_t i = 0; i < ne * nd; ++i) { rc_list[(size_t)pos[dof[i]]++] = (int)i; }
+      release_global(h);
+      return h->fail(LFASM_E_UNSUPPORTED, "row length %d > 65535", maxrow);
    }
-   LF_HIP(upload(h, h->Ip, I, (size_t)n + 1));
-   LF_HIP(upload(h, h->Jc, Jcol, (size_t)nnz));
-   LF_HIP(upload(h, h->rc_ptr, rc_ptr.data(), rc_ptr.size()));
-   LF_HIP(upload(h, h->rc_list, rc_list.data(), rc_list.size()));
+   h->maxrow = maxrow;
    std::vector<unsigned char> ess((size_t)n, 0);
    if (is_ess) { std::memcpy(ess.data(), is_ess, (size_t)n); }
    LF_HIP(upload(h, h->ess, ess.data(), ess.size()));
@@ -1048,7 +1048,7 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
       eflag[i] = (unsigned char)((sign[i] < 0.0 ? 1 : 0) | (ess[(size_t)dof[i]] ? 2 : 0));
    }
    LF_HIP(upload(h, h->eflag, eflag.data(), eflag.size()));
-   LF_HIP(hipMalloc((void **)&h->vals, (size_t)nnz * sizeof(double2)));
+   LF_HIP(hipMalloc((void **)&h->vals, (size_t)h->nnz * sizeof(double2)));
    LF_HIP(hipMalloc((void **)&h->slot, ne * nd * nd * sizeof(uint16_t)));
    LF_HIP(hipMalloc((void **)&h->bad, sizeof(int)));
    LF_HIP(hipMemsetAsync(h->bad, 0, sizeof(int), h->stream));
@@ -1071,6 +1071,122 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
       if (rc != LFASM_OK) { release_global(h); return rc; }
    }
    h->global_built = true;
+   return LFASM_OK;
+}
+
+std::vector<int> download_dof(lfasm_handle *h, hipError_t &e)
+{
+   std::vector<int> dof((size_t)h->ne * h->nd);
+   e = hipMemcpy(dof.data(), h->dof, dof.size() * sizeof(int), hipMemcpyDeviceToHost);
+   return dof;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t nnz,
+                       const unsigned char *is_ess)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   if (!I || !Jcol || nnz <= 0 || nnz > INT32_MAX)
+   {
+      return h->fail(LFASM_E_ARG, "CSR pattern missing or nnz out of int32 range");
+   }
+   const int n = h->height;
+   if (I[0] != 0 || I[n] != nnz) { return h->fail(LFASM_E_ARG, "I[0] != 0 or I[n] != nnz"); }
+   for (int r = 0; r < n; ++r)
+   {
+      if (I[r + 1] < I[r]) { return h->fail(LFASM_E_ARG, "I not monotone at row %d", r); }
+   }
+   for (int64_t k = 0; k < nnz; ++k)
+   {
+      if (Jcol[k] < 0 || Jcol[k] >= n) { return h->fail(LFASM_E_ARG, "Jcol[%lld] out of range", (long long)k); }
+   }
+   LF_HIP(hipSetDevice(h->device));
+   release_global(h);
+   h->nnz = nnz;
+   hipError_t e = hipSuccess;
+   const std::vector<int> dof = download_dof(h, e);
+   LF_HIP(e);
+   std::vector<int> rc_ptr, rc_list;
+   host_contributors(n, dof, rc_ptr, rc_list);
+   LF_HIP(upload(h, h->Ip, I, (size_t)n + 1));
+   LF_HIP(upload(h, h->Jc, Jcol, (size_t)nnz));
+   LF_HIP(upload(h, h->rc_ptr, rc_ptr.data(), rc_ptr.size()));
+   LF_HIP(upload(h, h->rc_list, rc_list.data(), rc_list.size()));
+   return finish_global(h, I, dof, rc_ptr, rc_list, is_ess);
+}
+
+int lfasm_build_global_device(lfasm_handle *h, const unsigned char *is_ess)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   release_global(h);
+   const int n = h->height, nd = h->nd;
+   hipError_t e = hipSuccess;
+   const std::vector<int> dof = download_dof(h, e);
+   LF_HIP(e);
+   std::vector<int> rc_ptr, rc_list;
+   host_contributors(n, dof, rc_ptr, rc_list);
+   int maxc = 0;
+   for (int r = 0; r < n; ++r) { maxc = std::max(maxc, rc_ptr[(size_t)r + 1] - rc_ptr[r]); }
+   int cap = 64;
+   while (cap < maxc * nd) { cap <<= 1; }
+   const size_t shmem = (size_t)PAT_WAVES * cap * sizeof(int);
+   if (shmem > h->lds_max)
+   {
+      return h->fail(LFASM_E_UNSUPPORTED, "%d contributors x %d dofs per row exceed the LDS sort",
+                     maxc, nd);
+   }
+   LF_HIP(upload(h, h->rc_ptr, rc_ptr.data(), rc_ptr.size()));
+   LF_HIP(upload(h, h->rc_list, rc_list.data(), rc_list.size()));
+   LF_HIP(hipMalloc((void **)&h->Ip, ((size_t)n + 1) * sizeof(int)));
+   LF_HIP(hipFuncSetAttribute((const void *)k_csr_rows,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+   const dim3 grid((unsigned)((n + PAT_WAVES - 1) / PAT_WAVES));
+   LF_HIP(ev_begin(h, T_PATTERN));
+   hipLaunchKernelGGL(k_csr_rows, grid, dim3(64 * PAT_WAVES), shmem, h->stream, n, nd, h->dof,
+                      h->rc_ptr, h->rc_list, cap, 0, h->Ip, (int *)nullptr);
+   LF_HIP(hipGetLastError());
+   // row pointers: exclusive scan of the counts on the host (n + 1 ints)
+   std::vector<int> I((size_t)n + 1, 0);
+   LF_HIP(hipMemcpyAsync(I.data() + 1, h->Ip, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                         h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   int64_t nnz = 0;
+   for (int r = 0; r < n; ++r)
+   {
+      nnz += I[(size_t)r + 1];
+      if (nnz > INT32_MAX) { release_global(h); return h->fail(LFASM_E_UNSUPPORTED, "nnz > int32"); }
+      I[(size_t)r + 1] = (int)nnz;
+   }
+   h->nnz = nnz;
+   LF_HIP(hipMemcpyAsync(h->Ip, I.data(), I.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+   LF_HIP(hipMalloc((void **)&h->Jc, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
+   hipLaunchKernelGGL(k_csr_rows, grid, dim3(64 * PAT_WAVES), shmem, h->stream, n, nd, h->dof,
+                      h->rc_ptr, h->rc_list, cap, 1, h->Ip, h->Jc);
+   LF_HIP(hipGetLastError());
+   LF_HIP(ev_end(h, T_PATTERN));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   return finish_global(h, I.data(), dof, rc_ptr, rc_list, is_ess);
+}
+
+int lfasm_download_pattern(lfasm_handle *h, int *I, int *Jcol)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   if (I)
+   {
+      LF_HIP(hipMemcpyAsync(I, h->Ip, ((size_t)h->height + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                            h->stream));
+   }
+   if (Jcol)
+   {
+      LF_HIP(hipMemcpyAsync(Jcol, h->Jc, (size_t)h->nnz * sizeof(int), hipMemcpyDeviceToHost,
+                            h->stream));
+   }
+   LF_HIP(hipStreamSynchronize(h->stream));
    return LFASM_OK;
 }
 
@@ -1191,14 +1307,48 @@ int lfasm_download_values(lfasm_handle *h, double *re, double *im)
 {
    if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
    LF_HIP(hipSetDevice(h->device));
-   std::vector<double2> v((size_t)h->nnz);
-   LF_HIP(hipMemcpyAsync(v.data(), h->vals, v.size() * sizeof(double2), hipMemcpyDeviceToHost,
-                         h->stream));
-   LF_HIP(hipStreamSynchronize(h->stream));
-   for (size_t k = 0; k < v.size(); ++k)
+   // chunks of 64 MB through two pinned buffers: the copy of chunk k
+   // overlaps the host-side re/im split of chunk k-1 (threads)
+   constexpr size_t CH = (size_t)1 << 22;
+   for (int k = 0; k < 2; ++k)
    {
-      if (re) { re[k] = v[k].x; }
-      if (im) { im[k] = v[k].y; }
+      if (!h->pin[k]) { LF_HIP(hipHostMalloc((void **)&h->pin[k], CH * sizeof(double2))); }
+      if (!h->pin_ev[k]) { LF_HIP(hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming)); }
+   }
+   const size_t nnz = (size_t)h->nnz;
+   const size_t nch = (nnz + CH - 1) / CH;
+   const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+   auto split = [&](size_t c) {
+      const double2 *src = h->pin[c & 1];
+      const size_t off = c * CH, len = std::min(CH, nnz - off);
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < nthr; ++t)
+      {
+         pool.emplace_back([=]() {
+            const size_t a = len * t / nthr, b = len * (t + 1) / nthr;
+            for (size_t k = a; k < b; ++k)
+            {
+               if (re) { re[off + k] = src[k].x; }
+               if (im) { im[off + k] = src[k].y; }
+            }
+         });
+      }
+      for (auto &th : pool) { th.join(); }
+   };
+   for (size_t c = 0; c <= nch; ++c)
+   {
+      if (c < nch)
+      {
+         const size_t off = c * CH, len = std::min(CH, nnz - off);
+         LF_HIP(hipMemcpyAsync(h->pin[c & 1], h->vals + off, len * sizeof(double2),
+                               hipMemcpyDeviceToHost, h->stream));
+         LF_HIP(hipEventRecord(h->pin_ev[c & 1], h->stream));
+      }
+      if (c > 0)
+      {
+         LF_HIP(hipEventSynchronize(h->pin_ev[(c - 1) & 1]));
+         split(c - 1);
+      }
    }
    return LFASM_OK;
 }

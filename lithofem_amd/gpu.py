@@ -117,6 +117,11 @@ def lib():
         "lfasm_assemble_rhs": ([vp, i], i),
         "lfasm_download_rhs": ([vp, vp, vp], i),
         "lfasm_eps_sensitivity": ([vp, i, vp, vp, vp, i, vp], i),
+        "lfasm_set_tensor_path": ([vp, i], i),
+        "lfasm_step_capture": ([vp, i, i, i, i], i),
+        "lfasm_step_launch": ([vp, i], i),
+        "lfasm_build_global_device": ([vp, vp], i),
+        "lfasm_download_pattern": ([vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -263,6 +268,23 @@ class AsmGpuGlobal:
         self._check(lib().lfasm_build_global(self._h, _p(I), _p(Jcol), len(Jcol), _p(ess)))
         self.nnz = len(Jcol)
 
+    def build_global_device(self, ess_list=None):
+        """CSR pattern built on the device (replaces the host
+        build_csr_symbolic + build_global pair)."""
+        ess = np.zeros(self.n, dtype=np.uint8)
+        if ess_list is not None and len(ess_list):
+            ess[np.asarray(ess_list)] = 1
+        self._check(lib().lfasm_build_global_device(self._h, _p(ess)))
+        nnz = np.zeros(1, dtype=np.int64)
+        self._check(lib().lfasm_sizes(self._h, None, None, None, _p(nnz)))
+        self.nnz = int(nnz[0])
+
+    def download_pattern(self):
+        I = np.empty(self.n + 1, dtype=np.int32)
+        J = np.empty(self.nnz, dtype=np.int32)
+        self._check(lib().lfasm_download_pattern(self._h, _p(I), _p(J)))
+        return I, J
+
     # -- AsmGpu::ComputeLocal
     def compute_local(self, mask_cc=CURLCURL, mask_mass=MASS_ALL, apply_te=True,
                       kernel=KERNEL_AUTO):
@@ -348,11 +370,11 @@ class AsmGpuGlobal:
         return out
 
     def timings(self) -> dict:
-        ms = np.zeros(8)
-        self._check(lib().lfasm_timings(self._h, _p(ms), 8))
+        ms = np.zeros(9)
+        self._check(lib().lfasm_timings(self._h, _p(ms), 9))
         return dict(local_ms=ms[0], scatter_ms=ms[1], build_ms=ms[2], batch_ms=ms[3],
                     adjoint_ms=ms[4], local_tensor_ms=ms[5], local_quad_ms=ms[6],
-                    step_ms=ms[7])
+                    step_ms=ms[7], pattern_ms=ms[8])
 
     # -- captured step (HIP graph)
     def step_capture(self, mask_cc=CURLCURL, mask_mass=MASS_ALL, apply_te=True, eliminate=True):

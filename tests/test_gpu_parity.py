@@ -237,3 +237,35 @@ def test_state_errors():
         assert ei.value.code == gpu.E_PATTERN
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("name,p,renumber", [("grating", 1, False), ("grating", 3, False),
+                                             ("hole", 2, False), ("hole", 3, True)])
+def test_device_csr_pattern(name, p, renumber):
+    """The pattern built on the device equals the host symbolic phase bit for
+    bit, and the assembled system through it equals the host-pattern one."""
+    import dataclasses
+    from lithofem_amd.fespace import csr_symbolic
+    c, ad, I, J = case(name, p, 6.0, "small")
+    if renumber:
+        rng = np.random.default_rng(31)
+        perm = rng.permutation(ad.height).astype(np.int32)
+        ad = dataclasses.replace(ad, dof=np.ascontiguousarray(perm[ad.dof]),
+                                 ess=np.sort(perm[ad.ess]).astype(np.int32))
+        I, J = csr_symbolic(ad.dof, ad.height)
+    g = _ctx(c, ad)
+    try:
+        g.build_global(I, J, ad.ess)
+        g.compute_local()
+        g.scatter_eliminate(True)
+        ref = g.download_values()
+        g.build_global_device(ad.ess)
+        assert g.timings()["pattern_ms"] > 0
+        Id, Jd = g.download_pattern()
+        assert np.array_equal(Id, I) and np.array_equal(Jd, J)
+        g.compute_local()
+        g.scatter_eliminate(True)
+        got = g.download_values()
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    finally:
+        g.close()
