@@ -217,3 +217,29 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_matrixmarket_round_trip(tmp_path):
+    """The oracle's eliminated system and an RHS written in the reference's
+    MatrixMarket format read back bit for bit (17 significant digits)."""
+    from lithofem_amd import mmio
+    from oracle import oracle as O
+    c = contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=6.0)
+    ad = build_asm_data(c.mesh, 2)
+    I, J = csr_symbolic(ad.dof, ad.height)
+    orc = O.Oracle(ad, c.problem, I, J)
+    vre, vim = orc.assemble()
+    is_ess = np.zeros(ad.height, np.uint8)
+    is_ess[ad.ess] = 1
+    vre, vim = O.eliminate(I, J, is_ess, vre, vim)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(ad.height) * 1e-7 + 1j * rng.standard_normal(ad.height)
+    prefix = str(tmp_path / "sys")
+    mmio.export_system(prefix, I, J, vre, vim, b)
+    head = open(prefix + ".mtx").read(200).splitlines()
+    assert head[0] == "%%MatrixMarket matrix coordinate complex general"
+    assert head[1] == f"{ad.height} {ad.height} {len(J)}"
+    I2, J2, r2, i2 = mmio.read_system(prefix + ".mtx")
+    assert np.array_equal(I2, I) and np.array_equal(J2, J)
+    assert np.array_equal(r2, vre) and np.array_equal(i2, vim)
+    assert np.array_equal(mmio.read_vector(prefix + ".rhs.mtx"), b)
