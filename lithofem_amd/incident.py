@@ -12,8 +12,10 @@ time convention exp(-i w t), fields as envelopes (exp(i k_par . r)
 factored out).  Continuity of tangential E and H at every interface is
 solved directly as one small linear system per polarisation.
 
-Polarisation: s = E along s_hat = (-sin phi, cos phi, 0); p = H along
-s_hat.  Jones vectors combine the two linearly.
+Polarisation (the reference's convention, src/lithofem/incident.py:82-97):
+s = E along e_s = (-sin phi, cos phi, 0); p = E along e_p = k_hat x e_s
+(k_hat the downward incident direction), both with unit amplitude and zero
+phase at z_max.  Jones vectors combine the two linearly.
 """
 
 from __future__ import annotations
@@ -37,7 +39,10 @@ def _solve_stack(k0, kpar, eps, zs, pol):
     flux = kz E'; TM (p): field = H_s, flux = kz/eps H'."""
     L = len(eps)
     kz = [_kz(k0, e, kpar) for e in eps]
-    zref = [zs[1]] + [zs[j] for j in range(1, L)]        # bottom layer: its top
+    # bottom layer referenced at its top, the top layer at z_max (where the
+    # incident wave has unit amplitude and zero phase), the others at their
+    # bottom interface
+    zref = [zs[1]] + [zs[j] for j in range(1, L - 1)] + ([zs[L]] if L > 1 else [])
     M = np.zeros((2 * L, 2 * L), dtype=np.complex128)
     rhs = np.zeros(2 * L, dtype=np.complex128)
     row = 0
@@ -82,8 +87,10 @@ def slab_waves(problem, theta_deg: float, phi_deg: float, jones=(1.0, 0.0),
     _, _, amp_p = _solve_stack(k0, kpar, eps, zs, "p")
     for j in range(len(slabs)):
         e = eps[j]
-        up_p = (kz[j] / (k0 * e)) * p_hat - (kpar / (k0 * e)) * z_hat
-        dn_p = -(kz[j] / (k0 * e)) * p_hat - (kpar / (k0 * e)) * z_hat
+        # E of the p waves (H along -e_s for the downward wave), so that the
+        # incident wave in vacuum is exactly e_p = cos(theta) p_hat + sin(theta) z_hat
+        up_p = -(kz[j] / (k0 * e)) * p_hat + (kpar / (k0 * e)) * z_hat
+        dn_p = (kz[j] / (k0 * e)) * p_hat + (kpar / (k0 * e)) * z_hat
         A[j] = js * amp_s[j, 0] * s_hat + jp * amp_p[j, 0] * up_p
         B[j] = js * amp_s[j, 1] * s_hat + jp * amp_p[j, 1] * dn_p
         out[j, 0] = zref[j]

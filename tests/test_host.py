@@ -243,3 +243,93 @@ def test_matrixmarket_round_trip(tmp_path):
     assert np.array_equal(I2, I) and np.array_equal(J2, J)
     assert np.array_equal(r2, vre) and np.array_equal(i2, vim)
     assert np.array_equal(mmio.read_vector(prefix + ".rhs.mtx"), b)
+
+
+def _stack(slabs, wavelength=13.5):
+    from types import SimpleNamespace
+    return SimpleNamespace(k0=2 * math.pi / wavelength, slabs=slabs)
+
+
+def _fields(W, j, z):
+    from lithofem_amd.incident import evaluate
+    return evaluate(W, j, np.atleast_1d(z))[0]
+
+
+@pytest.mark.parametrize("pol", [(1.0, 0.0), (0.0, 1.0)])
+def test_incident_uniform_background_is_bare_wave(pol):
+    """Vacuum everywhere: no reflected wave, |E| = 1, phase advance
+    exp(-i qz dz) (the reference's tests/test_incident.py:78-97)."""
+    pb = _stack([(0.0, 60.0, 1.0), (60.0, 120.0, 1.0)])
+    W = slab_waves(pb, 20.0, 30.0, pol)
+    k0 = pb.k0
+    kp = k0 * math.sin(math.radians(20.0))
+    qz = math.sqrt(k0 * k0 - kp * kp)
+    z = np.linspace(1.0, 119.0, 57)
+    e = np.array([_fields(W, 0 if zz < 60 else 1, zz) for zz in z])
+    assert np.max(np.abs(np.linalg.norm(e, axis=1) - 1.0)) < 1e-12
+    ratio = e[1:] / e[:-1]
+    ok = np.abs(e[:-1]) > 1e-8
+    assert np.max(np.abs(ratio[ok] - np.exp(-1j * qz * (z[1] - z[0])))) < 1e-10
+    assert np.abs(W[1][6:12]).max() < 1e-14          # no upward wave on top
+
+
+def test_incident_polarisation_conventions():
+    """s at normal incidence: E = +y, amplitude 1 at z_max; oblique p: E at
+    z_max = e_p = k_hat x e_s (tests/test_incident.py:117-140)."""
+    pb = _stack([(0.0, 60.0, 1.0), (60.0, 120.0, 1.0)])
+    e = _fields(slab_waves(pb, 0.0, 0.0, (1.0, 0.0)), 1, 120.0)
+    assert np.allclose(e, [0.0, 1.0, 0.0], atol=1e-13)
+    th, ph = math.radians(30.0), math.radians(40.0)
+    k_hat = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), -math.cos(th)])
+    e_s = np.cross([0.0, 0.0, 1.0], k_hat)
+    e_s /= np.linalg.norm(e_s)
+    e_p = np.cross(k_hat, e_s)
+    e = _fields(slab_waves(pb, 30.0, 40.0, (0.0, 1.0)), 1, 120.0)
+    assert np.linalg.norm(e - e_p) < 1e-12
+    e = _fields(slab_waves(pb, 30.0, 40.0, (1.0, 0.0)), 1, 120.0)
+    assert np.linalg.norm(e - e_s) < 1e-12
+
+
+@pytest.mark.parametrize("pol", ["s", "p"])
+@pytest.mark.parametrize("eps2", [2.13, (0.95 + 0.031j) ** 2])
+def test_incident_fresnel_and_energy(pol, eps2):
+    """Vacuum over a half space: reflection = the Fresnel coefficient, and
+    the reflected plus transmitted flux at the interface equals the
+    incident flux (R + T = 1, lossless or absorbing substrate)."""
+    pb = _stack([(0.0, 60.0, eps2), (60.0, 120.0, 1.0)])
+    theta = 15.0
+    W = slab_waves(pb, theta, 0.0, (1.0, 0.0) if pol == "s" else (0.0, 1.0))
+    k0 = pb.k0
+    kp = k0 * math.sin(math.radians(theta))
+    q1 = np.sqrt(complex(k0 * k0 - kp * kp))
+    q2 = np.sqrt(complex(k0 * k0 * eps2 - kp * kp))
+    if q2.imag < 0:
+        q2 = -q2
+    top = W[1]
+    A = top[6:12:2] + 1j * top[7:12:2]       # reflected (upward) wave at z_max
+    B = top[12:18:2] + 1j * top[13:18:2]     # incident wave at z_max
+    # both referenced at z_max = 120; the interface is at z = 60
+    ph = np.exp(2j * q1 * (60.0 - 120.0))
+    if pol == "s":
+        r = (q1 - q2) / (q1 + q2)
+        assert abs(A[1] / B[1] * ph - r) < 1e-12
+    else:
+        r = (eps2 * q1 - q2) / (eps2 * q1 + q2)     # H_s ratio of the TM wave
+        # E_p of the waves; |H| ratio = |E| ratio in vacuum (|ph| = 1)
+        assert abs(np.linalg.norm(A) - abs(r)) < 1e-12
+    # Poynting flux S_z = Re(E x conj(H))_z / 2 with H = K x E / k0 (mu = 1)
+    p_hat = np.array([1.0, 0.0, 0.0])
+    z_hat = np.array([0.0, 0.0, 1.0])
+
+    def flux(E, q):
+        K = kp * p_hat + q * z_hat
+        H = np.cross(K, E) / k0
+        return 0.5 * np.real(np.cross(E, np.conj(H))[2])
+
+    bot = W[0]
+    Bt = bot[12:18:2] + 1j * bot[13:18:2]    # transmitted wave at the interface
+    S_in = flux(B, -q1)                       # downward: negative
+    R = -flux(A, q1) / S_in
+    T = flux(Bt, -q2) / S_in
+    # flux continuity at the interface (absorption happens below it)
+    assert abs(R + T - 1.0) < 1e-12
