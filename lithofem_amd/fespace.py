@@ -155,9 +155,26 @@ def build_asm_data(mesh: TetMesh, p: int) -> AsmData:
     v0 = x[:, 0].copy()
 
     # --- essential DOFs: edges and faces lying in a PEC boundary layer
-    vk = mesh.vert_k
     ess_mask = np.zeros(height, dtype=bool)
-    for layer in mesh.bdr_layers:
+    if mesh.bdr_faces is not None:
+        # boundary triangles given explicitly: their element faces and the
+        # three edges of each
+        bkeys = np.sort(np.asarray(mesh.bdr_faces, dtype=np.int64), axis=1)
+        bkeys = (bkeys[:, 0] * nv + bkeys[:, 1]) * nv + bkeys[:, 2]
+        on_face = np.isin(fkey, bkeys)                      # (ne, 4)
+        for f in range(4):
+            sel = on_face[:, f]
+            if not np.any(sel):
+                continue
+            fv = set(FACES[f])
+            for le_i, (va, vb) in enumerate(EDGES):
+                if va in fv and vb in fv:
+                    ess_mask[dof[sel, le_i * p:(le_i + 1) * p].ravel()] = True
+            if p >= 2:
+                k0 = 6 * p + f * 2 * nfp
+                ess_mask[dof[sel, k0:k0 + 2 * nfp].ravel()] = True
+    vk = mesh.vert_k
+    for layer in (mesh.bdr_layers if mesh.bdr_faces is None else ()):
         on_e = (vk[le[..., 0]] == layer) & (vk[le[..., 1]] == layer)
         for le_i in range(6):
             sel = on_e[:, le_i]

@@ -73,8 +73,9 @@ def slab_waves(problem, theta_deg: float, phi_deg: float, jones=(1.0, 0.0),
     slabs = problem.slabs
     eps = [complex(s[2]) for s in slabs]
     zs = [slabs[0][0]] + [s[1] for s in slabs[:-1]] + [slabs[-1][1]]
-    n_top = np.sqrt(complex(eps[-1])).real
-    kpar = k0 * n_top * math.sin(math.radians(theta_deg))
+    # k_par = k0 sin(theta) whatever the top medium (the reference's
+    # convention, src/lithofem/config.py:472-473)
+    kpar = k0 * math.sin(math.radians(theta_deg))
     cphi, sphi = math.cos(math.radians(phi_deg)), math.sin(math.radians(phi_deg))
     s_hat = np.array([-sphi, cphi, 0.0])
     p_hat = np.array([cphi, sphi, 0.0])

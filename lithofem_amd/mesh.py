@@ -33,6 +33,9 @@ class TetMesh:
     nz: int                     # number of z cells (vertex layers 0..nz)
     bdr_layers: tuple = (0,)    # z-layer indices that carry PEC boundaries
     meta: dict = field(default_factory=dict)
+    # PEC boundary triangles as identified vertex triples (meshes read from
+    # files); when set they define the essential DOFs instead of bdr_layers
+    bdr_faces: np.ndarray | None = None
 
     @property
     def ne(self) -> int:
