@@ -98,6 +98,10 @@ struct lfasm_handle {
    int *pelem = nullptr, *prc_ptr = nullptr, *prc_list = nullptr;
    double2 *Mhat = nullptr, *var_vals = nullptr;
    std::vector<int> pelem_host;
+   std::vector<int> dof_host;         // element dof table (host copy)
+   int pmaxc = 0;                     // most parameter contributors of one row
+   double2 *d_deps = nullptr;         // eps offsets of the variants (device)
+   size_t d_deps_cap = 0;
    // incident fields / RHS
    int inc_nrhs = 0, inc_nslab = 0;
    SlabWaveDev *waves = nullptr;
@@ -200,7 +204,8 @@ void release_global(lfasm_handle *h)
    h->fused_ok = false;
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
-   dfree(h->rhs);
+   dfree(h->rhs); dfree(h->d_deps);
+   h->d_deps_cap = 0;
    h->nparam = h->nvar = 0;
    h->global_built = false;
 }
@@ -558,6 +563,8 @@ TensorParams tensor_params(lfasm_handle *h, const LocalParams &P)
    T.use_klk = (P.mflags >> 3) & 1;
    T.use_cross = (P.mflags >> 1) & 1;
    T.apply_te = P.apply_te;
+   T.eps_unit = P.eps_unit;
+   T.out_compact = 0;
    T.Ae = P.Ae;
    return T;
 }
@@ -876,6 +883,7 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
       LF_HIP(upload(h, h->v0, m0.v0, ne * 3));
       LF_HIP(upload(h, h->attr, m0.attr, ne));
       LF_HIP(upload(h, h->dof, m0.dof, ne * nd));
+      h->dof_host.assign(m0.dof, m0.dof + ne * nd);
       LF_HIP(upload(h, h->sign, m0.sign, ne * nd));
       if (m0.p >= 2) { LF_HIP(upload(h, h->fo, m0.fo, ne * 4)); }
       if (m0.eps_elem) { LF_HIP(upload(h, h->eps_elem, (const double2 *)m0.eps_elem, ne)); }
@@ -917,6 +925,7 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
       LF_HIP(upload(h, h->v0, v0.data(), v0.size()));
       LF_HIP(upload(h, h->attr, attr.data(), attr.size()));
       LF_HIP(upload(h, h->dof, dof.data(), dof.size()));
+      h->dof_host = dof;
       LF_HIP(upload(h, h->sign, sign.data(), sign.size()));
       if (m0.p >= 2) { LF_HIP(upload(h, h->fo, fo.data(), fo.size())); }
       LF_HIP(upload(h, h->pid, pid.data(), pid.size()));
@@ -1074,11 +1083,10 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
    return LFASM_OK;
 }
 
-std::vector<int> download_dof(lfasm_handle *h, hipError_t &e)
+std::vector<int> host_dof(lfasm_handle *h, hipError_t &e)
 {
-   std::vector<int> dof((size_t)h->ne * h->nd);
-   e = hipMemcpy(dof.data(), h->dof, dof.size() * sizeof(int), hipMemcpyDeviceToHost);
-   return dof;
+   e = hipSuccess;
+   return h->dof_host;
 }
 
 }  // namespace
@@ -1107,7 +1115,7 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
    release_global(h);
    h->nnz = nnz;
    hipError_t e = hipSuccess;
-   const std::vector<int> dof = download_dof(h, e);
+   const std::vector<int> dof = host_dof(h, e);
    LF_HIP(e);
    std::vector<int> rc_ptr, rc_list;
    host_contributors(n, dof, rc_ptr, rc_list);
@@ -1125,7 +1133,7 @@ int lfasm_build_global_device(lfasm_handle *h, const unsigned char *is_ess)
    release_global(h);
    const int n = h->height, nd = h->nd;
    hipError_t e = hipSuccess;
-   const std::vector<int> dof = download_dof(h, e);
+   const std::vector<int> dof = host_dof(h, e);
    LF_HIP(e);
    std::vector<int> rc_ptr, rc_list;
    host_contributors(n, dof, rc_ptr, rc_list);
@@ -1439,6 +1447,49 @@ int lfasm_timings(const lfasm_handle *hc, double *ms, int n)
 
 // ---- eps variants ---------------------------------------------------------------
 
+}  // extern "C"
+
+namespace {
+
+// Host-to-device copy of pageable memory through the handle's two pinned
+// 64 MB buffers: the host copy of chunk k+1 (threads) overlaps the DMA of
+// chunk k.  Ordered on the handle's stream.
+int upload_pinned(lfasm_handle *h, void *dst, const void *src, size_t bytes)
+{
+   constexpr size_t CH = ((size_t)1 << 22) * sizeof(double2);
+   for (int k = 0; k < 2; ++k)
+   {
+      if (!h->pin[k]) { LF_HIP(hipHostMalloc((void **)&h->pin[k], CH)); }
+      if (!h->pin_ev[k]) { LF_HIP(hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming)); }
+   }
+   const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+   const size_t nch = (bytes + CH - 1) / CH;
+   for (size_t c = 0; c < nch; ++c)
+   {
+      const size_t off = c * CH, len = std::min(CH, bytes - off);
+      char *pin = reinterpret_cast<char *>(h->pin[c & 1]);
+      LF_HIP(hipEventSynchronize(h->pin_ev[c & 1]));      // its previous DMA is done
+      const char *s = reinterpret_cast<const char *>(src) + off;
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < nthr; ++t)
+      {
+         pool.emplace_back([=]() {
+            const size_t a = len * t / nthr, b = len * (t + 1) / nthr;
+            std::memcpy(pin + a, s + a, b - a);
+         });
+      }
+      for (auto &th : pool) { th.join(); }
+      LF_HIP(hipMemcpyAsync(reinterpret_cast<char *>(dst) + off, pin, len, hipMemcpyHostToDevice,
+                            h->stream));
+      LF_HIP(hipEventRecord(h->pin_ev[c & 1], h->stream));
+   }
+   return LFASM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems)
 {
    if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
@@ -1451,14 +1502,12 @@ int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems)
       }
    }
    LF_HIP(hipSetDevice(h->device));
-   dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
-   h->nvar = 0;
+   dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat);
    const size_t nd = (size_t)h->nd;
    h->pelem_host.assign(elems, elems + nparam);
    LF_HIP(upload(h, h->pelem, elems, (size_t)nparam));
    // per-row parameter contributors, ascending element order
-   std::vector<int> dof((size_t)h->ne * nd);
-   LF_HIP(hipMemcpy(dof.data(), h->dof, dof.size() * sizeof(int), hipMemcpyDeviceToHost));
+   const std::vector<int> &dof = h->dof_host;
    std::vector<int> ptr((size_t)h->height + 1, 0), lst((size_t)nparam * nd);
    for (int i = 0; i < nparam; ++i)
       for (size_t a = 0; a < nd; ++a) { ptr[(size_t)dof[(size_t)elems[i] * nd + a] + 1]++; }
@@ -1471,15 +1520,36 @@ int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems)
             lst[(size_t)pos[(size_t)dof[(size_t)elems[i] * nd + a]]++] = (int)(i * nd + a);
          }
    }
+   h->pmaxc = 0;
+   for (int r = 0; r < h->height; ++r) { h->pmaxc = std::max(h->pmaxc, ptr[(size_t)r + 1] - ptr[(size_t)r]); }
    LF_HIP(upload(h, h->prc_ptr, ptr.data(), ptr.size()));
    LF_HIP(upload(h, h->prc_list, lst.data(), lst.size()));
    LF_HIP(hipMalloc((void **)&h->Mhat, (size_t)nparam * nd * nd * sizeof(double2)));
-   // unit-eps mass matrices (face-transformed) of the parameter elements
+   // unit-eps mass matrices (face-transformed) of the parameter elements:
+   // the reference-tensor kernel when every one of them has constant
+   // coefficients, else the quadrature kernels
    LocalParams P = local_params(h, 0, LFASM_MASS, 1, h->Mhat);
    P.nout = nparam;
    P.elist = h->pelem;
    P.eps_unit = 1;
-   LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1));
+   bool all_const = h->Rten != nullptr;
+   for (int i = 0; all_const && i < nparam; ++i) { all_const = h->eclass_host[(size_t)elems[i]] != 0; }
+   if (all_const)
+   {
+      TensorParams T = tensor_params(h, P);
+      T.nlist = nparam;
+      T.elist = h->pelem;
+      T.out_compact = 1;
+      const int epar = T.nb > 128 ? 1 : 256 / T.nb;
+      const dim3 grid((unsigned)((nparam + TEN_EPC - 1) / TEN_EPC),
+                      epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
+      hipLaunchKernelGGL(k_local_tensor, grid, dim3(256), 0, h->stream, T, epar);
+      LF_HIP(hipGetLastError());
+   }
+   else
+   {
+      LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1));
+   }
    LF_HIP(hipStreamSynchronize(h->stream));
    h->nparam = nparam;
    return LFASM_OK;
@@ -1496,8 +1566,18 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
       LF_HIP(hipMalloc((void **)&h->var_vals, (size_t)nvar * h->nnz * sizeof(double2)));
       h->nvar = nvar;
    }
-   double2 *d_deps = nullptr;
-   LF_HIP(upload(h, d_deps, (const double2 *)deps, (size_t)nvar * h->nparam));
+   const size_t ndeps = (size_t)nvar * h->nparam;
+   if (ndeps > h->d_deps_cap)
+   {
+      dfree(h->d_deps);
+      LF_HIP(hipMalloc((void **)&h->d_deps, ndeps * sizeof(double2)));
+      h->d_deps_cap = ndeps;
+   }
+   {
+      const int rc = upload_pinned(h, h->d_deps, deps, ndeps * sizeof(double2));
+      if (rc != LFASM_OK) { return rc; }
+   }
+   const double2 *d_deps = h->d_deps;
    VariantParams V;
    V.n = h->height;
    V.nd = h->nd;
@@ -1520,15 +1600,28 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    const size_t shmem = (size_t)4 * 2 * h->maxrow * sizeof(double2);
    hipError_t e = hipFuncSetAttribute((const void *)k_variant_rows,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   if (e == hipSuccess)
+   {
+      e = hipFuncSetAttribute((const void *)k_variant_rows_reg,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   }
    if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
    if (e == hipSuccess)
    {
-      hipLaunchKernelGGL(k_variant_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem, h->stream, V);
+      if (h->nd <= 64 && h->pmaxc <= VMAXC)
+      {
+         hipLaunchKernelGGL(k_variant_rows_reg, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
+                            h->stream, V);
+      }
+      else
+      {
+         hipLaunchKernelGGL(k_variant_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
+                            h->stream, V);
+      }
       e = hipGetLastError();
    }
    if (e == hipSuccess) { e = ev_end(h, T_BATCH); }
    if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
-   (void)hipFree(d_deps);
    LF_HIP(e);
    return LFASM_OK;
 }

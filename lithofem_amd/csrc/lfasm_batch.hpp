@@ -36,6 +36,97 @@ struct VariantParams {
    int eliminate, maxrow;
 };
 
+// Register-resident variant of k_variant_rows for nd <= 64 and at most
+// VMAXC parameter contributors per row: each lane keeps its column's
+// (signed) unit-eps entry, slot and use flag of every contributor, so the
+// Mhat rows are read once per row instead of once per variant; the
+// variants then only rescale and accumulate.  Same sums, same order.
+constexpr int VMAXC = 16;
+
+__global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
+{
+   extern __shared__ double2 vbuf[];
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + wave;
+   if (r >= V.n) { return; }
+   double2 *row0 = vbuf + (size_t)wave * 2 * V.maxrow;
+   double2 *buf = row0 + V.maxrow;
+   const int base = V.Ip[r];
+   const int len = V.Ip[r + 1] - base;
+   const int c0 = V.prc_ptr[r], c1 = V.prc_ptr[r + 1];
+   const bool row_fixed = c0 == c1 || (V.eliminate && V.is_ess[r]);
+   if (row_fixed)
+   {
+      for (int v = 0; v < V.nvar; ++v)
+      {
+         for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = V.base[base + s]; }
+      }
+      return;
+   }
+   for (int s = lane; s < len; s += 64) { row0[s] = V.base[base + s]; }
+   const int nd = V.nd;
+   const int nc = c1 - c0;                    // <= VMAXC (host-checked)
+   const unsigned char ess_bit = V.eliminate ? 2 : 0;
+   const int b = lane;
+   const bool live = b < nd;
+   double2 m[VMAXC];
+   int sl[VMAXC], pidx[VMAXC];
+   bool use[VMAXC];
+#pragma unroll
+   for (int k = 0; k < VMAXC; ++k)
+   {
+      m[k] = make_double2(0.0, 0.0);
+      sl[k] = 0;
+      pidx[k] = 0;
+      use[k] = false;
+      if (k < nc)
+      {
+         const int pa = V.prc_list[c0 + k];
+         const int pi = pa / nd, a = pa - pi * nd;
+         const int e = V.pelem[pi];
+         const int ea = e * nd + a;
+         pidx[k] = pi;
+         if (live)
+         {
+            const unsigned char fa = V.eflag[ea];
+            const unsigned char fb = V.eflag[(size_t)e * nd + b];
+            use[k] = (fb & ess_bit) == 0;
+            if (use[k])
+            {
+               sl[k] = V.slot[(size_t)ea * nd + b];
+               const double2 x = V.Mhat[((size_t)pi * nd + a) * nd + b];
+               m[k] = ((fa ^ fb) & 1) ? make_double2(-x.x, -x.y) : x;
+            }
+         }
+      }
+   }
+   for (int v = 0; v < V.nvar; ++v)
+   {
+      wave_sync_lds();
+      for (int s = lane; s < len; s += 64) { buf[s] = row0[s]; }
+      wave_sync_lds();
+      const double2 *dv = V.deps + (size_t)v * V.nparam;
+#pragma unroll
+      for (int k = 0; k < VMAXC; ++k)
+      {
+         if (k < nc)
+         {
+            const double2 t = cmul(dv[pidx[k]], m[k]);
+            if (use[k])
+            {
+               double2 x = buf[sl[k]];
+               x.x += t.x;
+               x.y += t.y;
+               buf[sl[k]] = x;
+            }
+            wave_sync_lds();
+         }
+      }
+      for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = buf[s]; }
+   }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
 {
    extern __shared__ double2 vbuf[];

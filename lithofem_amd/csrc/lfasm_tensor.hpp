@@ -100,6 +100,8 @@ struct TensorParams {
    double TInv[24];
    int use_cc, use_mass, use_klk, use_cross;
    int apply_te;
+   int eps_unit;            // 1: eps = 1 in the mass term (unit-eps mass matrices)
+   int out_compact;         // 1: list entry i writes Ae[i] (else Ae[elist[i]])
    double2 *Ae;
 };
 
@@ -116,7 +118,8 @@ __device__ inline void tensor_coef(const TensorParams &P, int e, double *c)
    const double det = P.detJ[e];
    const int pid = P.pid ? P.pid[e] : 0;
    const PmlDev pm = P.pml_tab[pid];
-   const double2 eps = P.eps_elem ? P.eps_elem[e] : P.eps_tab[P.eps_off[pid] + P.attr[e] - 1];
+   const double2 eps = P.eps_unit ? make_double2(1.0, 0.0)
+                       : P.eps_elem ? P.eps_elem[e] : P.eps_tab[P.eps_off[pid] + P.attr[e] - 1];
    const double k2 = pm.k0 * pm.k0;
    const double kx = pm.kx, ky = pm.ky;
    // -K^2 = |k|^2 I - k k^T for k = (kx, ky, 0)
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
    if (tid < cnt)
    {
       const int e = P.elist[c0 + tid];
-      sel[tid] = e;
+      sel[tid] = P.out_compact ? c0 + tid : e;
       tensor_coef(P, e, coef + tid * TEN_NC);
       if (P.p >= 2)
       {
