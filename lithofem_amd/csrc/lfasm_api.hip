@@ -91,7 +91,9 @@ struct lfasm_handle {
    int maxrow = 0;
    int *Ip = nullptr, *Jc = nullptr, *rc_ptr = nullptr, *rc_list = nullptr, *bad = nullptr;
    uint16_t *slot = nullptr;
+   int sld = 0;                       // slot row stride (entries)
    unsigned char *ess = nullptr, *eflag = nullptr;
+   uint64_t *emask = nullptr;         // nd <= 64: per element sign and essential bit masks
    double2 *vals = nullptr;
    // eps variants
    int nparam = 0, nvar = 0;
@@ -203,6 +205,7 @@ void release_global(lfasm_handle *h)
    dfree(h->rc_enc);
    h->fused_ok = false;
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
+   dfree(h->emask);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
    dfree(h->rhs); dfree(h->d_deps);
    h->d_deps_cap = 0;
@@ -518,6 +521,7 @@ hipError_t launch_fused_rows(lfasm_handle *h, int eliminate)
       F.Jc = h->Jc;
       F.eliminate = eliminate ? 1 : 0;
       F.slot = h->slot;
+      F.sld = h->sld;
       F.eflag = h->eflag;
       std::memcpy(F.TInv, h->TInv, sizeof(F.TInv));
       F.Ae = h->Ae;
@@ -658,9 +662,11 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.Ip = h->Ip;
    S.Jc = h->Jc;
    S.eflag = h->eflag;
+   S.emask = h->emask;
    S.rc_ptr = h->rc_ptr;
    S.rc_list = h->rc_list;
    S.slot = h->slot;
+   S.sld = h->sld;
    S.is_ess = h->ess;
    S.Ae = h->Ae;
    S.vals = h->vals;
@@ -1057,13 +1063,28 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
       eflag[i] = (unsigned char)((sign[i] < 0.0 ? 1 : 0) | (ess[(size_t)dof[i]] ? 2 : 0));
    }
    LF_HIP(upload(h, h->eflag, eflag.data(), eflag.size()));
+   if (nd <= 64)
+   {
+      std::vector<uint64_t> mask(2 * ne, 0);
+      for (size_t e = 0; e < ne; ++e)
+         for (size_t b = 0; b < nd; ++b)
+         {
+            const unsigned char f = eflag[e * nd + b];
+            if (f & 1) { mask[2 * e] |= 1ull << b; }
+            if (f & 2) { mask[2 * e + 1] |= 1ull << b; }
+         }
+      LF_HIP(upload(h, h->emask, mask.data(), mask.size()));
+   }
    LF_HIP(hipMalloc((void **)&h->vals, (size_t)h->nnz * sizeof(double2)));
-   LF_HIP(hipMalloc((void **)&h->slot, ne * nd * nd * sizeof(uint16_t)));
+   // rows unpadded: padding them to aligned 128-byte lines measured no
+   // faster (1.132 vs 1.135 ms at 1.35M DOF) for 42% more memory
+   h->sld = (int)nd;
+   LF_HIP(hipMalloc((void **)&h->slot, ne * nd * (size_t)h->sld * sizeof(uint16_t)));
    LF_HIP(hipMalloc((void **)&h->bad, sizeof(int)));
    LF_HIP(hipMemsetAsync(h->bad, 0, sizeof(int), h->stream));
    LF_HIP(ev_begin(h, T_BUILD));
-   hipLaunchKernelGGL(k_slot_map, dim3(2048), dim3(BLOCK), 0, h->stream, h->ne, h->nd, h->dof,
-                      h->Ip, h->Jc, h->slot, h->bad);
+   hipLaunchKernelGGL(k_slot_map, dim3(2048), dim3(BLOCK), 0, h->stream, h->ne, h->nd, h->sld,
+                      h->dof, h->Ip, h->Jc, h->slot, h->bad);
    LF_HIP(hipGetLastError());
    LF_HIP(ev_end(h, T_BUILD));
    int bad = 0;
@@ -1590,6 +1611,7 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    V.prc_list = h->prc_list;
    V.pelem = h->pelem;
    V.slot = h->slot;
+   V.sld = h->sld;
    V.is_ess = h->ess;
    V.Mhat = h->Mhat;
    V.deps = d_deps;

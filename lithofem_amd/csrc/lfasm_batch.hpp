@@ -27,7 +27,8 @@ struct VariantParams {
    const unsigned char *eflag;   // per (e, b): bit0 sign < 0, bit1 essential
    const int *prc_ptr, *prc_list;  // per row: param contributors pi*nd + a, ascending element
    const int *pelem;             // param index -> element
-   const uint16_t *slot;         // element slot map
+   const uint16_t *slot;         // element slot map (rows of sld entries)
+   int sld;
    const unsigned char *is_ess;
    const double2 *Mhat;          // nparam x nd x nd
    const double2 *deps;          // nvar x nparam
@@ -94,7 +95,7 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
             use[k] = (fb & ess_bit) == 0;
             if (use[k])
             {
-               sl[k] = V.slot[(size_t)ea * nd + b];
+               sl[k] = V.slot[(size_t)ea * V.sld + b];
                const double2 x = V.Mhat[((size_t)pi * nd + a) * nd + b];
                m[k] = ((fa ^ fb) & 1) ? make_double2(-x.x, -x.y) : x;
             }
@@ -167,7 +168,7 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
          {
             const unsigned char fb = V.eflag[(size_t)e * nd + b];
             if (fb & ess_bit) { continue; }
-            const int s = V.slot[(size_t)ea * nd + b];
+            const int s = V.slot[(size_t)ea * V.sld + b];
             const double2 m = V.Mhat[((size_t)pi * nd + a) * nd + b];
             double2 t = cmul(d, m);
             if ((fa ^ fb) & 1) { t.x = -t.x; t.y = -t.y; }

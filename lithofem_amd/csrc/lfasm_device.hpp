@@ -698,10 +698,10 @@ __global__ void k_te(int nd, int p, const int *__restrict__ fo, const int *__res
 }
 
 // ---- CSR slot map ----------------------------------------------------------
-// slot[(e*nd + a)*nd + b] = position of column dof[e][b] inside row
+// slot[(e*nd + a)*sld + b] = position of column dof[e][b] inside row
 // dof[e][a] (lower bound in the sorted row, as solver/asm_gpu.inc:447-453
 // and solver/asm_data.inc:393).  A missing column sets *bad.
-__global__ void k_slot_map(int ne, int nd, const int *__restrict__ dof,
+__global__ void k_slot_map(int ne, int nd, int sld, const int *__restrict__ dof,
                            const int *__restrict__ Ip, const int *__restrict__ Jc,
                            uint16_t *__restrict__ slot, int *__restrict__ bad)
 {
@@ -722,8 +722,9 @@ __global__ void k_slot_map(int ne, int nd, const int *__restrict__ dof,
          if (Jc[mid] < c) { lo = mid + 1; }
          else { hi = mid; }
       }
-      if (lo >= Ip[r + 1] || Jc[lo] != c) { *bad = 1; slot[idx] = 0; }
-      else { slot[idx] = (uint16_t)(lo - base); }
+      const size_t o = (e * nd + a) * (size_t)sld + b;   // rows of sld entries
+      if (lo >= Ip[r + 1] || Jc[lo] != c) { *bad = 1; slot[o] = 0; }
+      else { slot[o] = (uint16_t)(lo - base); }
    }
 }
 
@@ -738,6 +739,8 @@ struct ScatterParams {
    int n, nd;
    const int *Ip, *Jc;
    const unsigned char *eflag;  // per (e, b): bit0 sign < 0, bit1 dof essential
+   const uint64_t *emask;       // nd <= 64: per e, {sign < 0 bits, essential bits} (else NULL)
+   int sld;                     // slot row stride (entries)
    const int *rc_ptr, *rc_list;
    const uint16_t *slot;
    const unsigned char *is_ess;
@@ -762,7 +765,8 @@ constexpr int SCATTER_CG = 4;
 __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
 {
    extern __shared__ double2 rowbuf[];
-   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
    const int r = blockIdx.x * 4 + wave;
    if (r >= S.n) { return; }
    double2 *buf = rowbuf + (size_t)wave * S.maxrow;
@@ -795,10 +799,22 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
                {
                   const int ea = S.rc_list[c];          // e*nd + a
                   const int e = ea / nd;
-                  const unsigned char fa = S.eflag[ea];
-                  const unsigned char fb = S.eflag[(size_t)e * nd + b];
+                  unsigned char fa, fb;
+                  if (S.emask)
+                  {
+                     // two uniform 8-byte loads instead of a byte per lane
+                     const uint64_t sm = S.emask[2 * (size_t)e], em = S.emask[2 * (size_t)e + 1];
+                     const int a = ea - e * nd;
+                     fa = (unsigned char)((sm >> a) & 1);
+                     fb = (unsigned char)(((sm >> b) & 1) | (((em >> b) & 1) << 1));
+                  }
+                  else
+                  {
+                     fa = S.eflag[ea];
+                     fb = S.eflag[(size_t)e * nd + b];
+                  }
                   use[u] = (fb & ess_bit) == 0;
-                  sl[u] = S.slot[(size_t)ea * nd + b];
+                  sl[u] = S.slot[(size_t)ea * S.sld + b];
                   const dbl2v x = __builtin_nontemporal_load(
                      reinterpret_cast<const dbl2v *>(S.Ae) + (size_t)ea * nd + b);
                   const bool neg = ((fa ^ fb) & 1) != 0;

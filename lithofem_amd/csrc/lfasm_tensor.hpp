@@ -350,6 +350,7 @@ struct FusedParams {
    const int *Jc;
    int eliminate;
    const uint16_t *slot;
+   int sld;                    // slot row stride (entries)
    const unsigned char *eflag;
    double TInv[24];
    const double2 *Ae;
@@ -494,8 +495,8 @@ __global__ void __launch_bounds__(768) k_fused_rows(FusedParams F,
             const int ea = enc[k] & CE_MASK;
             const int e = ea / nd;
             fb[k] = F.eflag[(size_t)e * nd + bb];
-            sl0[k] = F.slot[(size_t)ea * nd + bb];
-            sl1[k] = pair ? F.slot[(size_t)(ea + 1) * nd + bb] : 0;
+            sl0[k] = F.slot[(size_t)ea * F.sld + bb];
+            sl1[k] = pair ? F.slot[(size_t)(ea + 1) * F.sld + bb] : 0;
             cl[k] = coef[(size_t)e * TEN_NC + min(lane, TEN_NC - 1)];
             ap0[k] = make_double2(0.0, 0.0);
             ap1[k] = make_double2(0.0, 0.0);
