@@ -288,8 +288,9 @@ void lfasm_free(void *p);
 /* Measured device ceilings (no reference counterpart): out[0] FP64 MFMA
  * TFLOP/s (v_mfma_f64_16x16x4f64 loop), out[1] FP64 VALU FMA TFLOP/s,
  * out[2] HBM copy GB/s (2 GiB read + 2 GiB write), out[3] reported clock
- * GHz, out[4] compute units.  Allocates 4 GiB of device memory while it
- * runs. */
+ * GHz, out[4] compute units, out[5] HBM write-only GB/s (2 GiB), out[6] HBM
+ * read-only GB/s (2 GiB).  `out` holds 7 doubles.  Allocates 4 GiB of device
+ * memory while it runs. */
 int lfasm_measure_peaks(int device, double *out);
 
 #ifdef __cplusplus

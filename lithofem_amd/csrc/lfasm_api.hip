@@ -2008,6 +2008,20 @@ int lfasm_measure_peaks(int device, double *out)
       out[3] = (double)prop.clockRate / 1e6;   // GHz (reported maximum)
       out[4] = ncu;
    }
+   {
+      hipLaunchKernelGGL(k_peak_write, dim3(ncu * 32), dim3(256), 0, s, b, n2);
+      ok(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_peak_write, dim3(ncu * 32), dim3(256), 0, s, b, n2);
+      ok(hipEventRecord(e1, s));
+      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
+      out[5] = (double)nbytes / (ms * 1e-3) / 1e9;
+      hipLaunchKernelGGL(k_peak_read, dim3(ncu * 32), dim3(256), 0, s, a, n2, dout);
+      ok(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_peak_read, dim3(ncu * 32), dim3(256), 0, s, a, n2, dout);
+      ok(hipEventRecord(e1, s));
+      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
+      out[6] = (double)nbytes / (ms * 1e-3) / 1e9;
+   }
 done:
    if (a) { (void)hipFree(a); }
    if (b) { (void)hipFree(b); }

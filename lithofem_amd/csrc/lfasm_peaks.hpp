@@ -58,4 +58,38 @@ __global__ void __launch_bounds__(256) k_peak_copy(const double2 *__restrict__ s
    }
 }
 
+// Write-only and read-only streams of double2, four per thread per
+// iteration (the write pattern of the element-matrix kernels, the read
+// side of the scatter).
+__global__ void __launch_bounds__(256) k_peak_write(double2 *__restrict__ dst, size_t n)
+{
+   const size_t stride = (size_t)gridDim.x * blockDim.x;
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 4 * stride)
+   {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+         const size_t j = i + u * stride;
+         if (j < n) { dst[j] = make_double2((double)j, 0.0); }
+      }
+   }
+}
+
+__global__ void __launch_bounds__(256) k_peak_read(const double2 *__restrict__ src, size_t n,
+                                                   double *__restrict__ out)
+{
+   const size_t stride = (size_t)gridDim.x * blockDim.x;
+   double acc = 0.0;
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 4 * stride)
+   {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+         const size_t j = i + u * stride;
+         if (j < n) { const double2 v = src[j]; acc += v.x + v.y; }
+      }
+   }
+   if (acc == 12345.678) { out[0] = acc; }   // keeps the loads; never true for zeros
+}
+
 }  // namespace lfasm

@@ -150,12 +150,13 @@ def csr_symbolic(dof: np.ndarray, height: int):
 
 
 def measure_peaks(device: int = 0) -> dict:
-    out = np.zeros(5)
+    out = np.zeros(7)
     rc = lib().lfasm_measure_peaks(device, _p(out))
     if rc != 0:
         raise AsmGpuError(rc, "measure_peaks failed")
     return dict(fp64_mfma_tflops=out[0], fp64_valu_tflops=out[1], hbm_copy_gbps=out[2],
-                clock_ghz=out[3], cus=int(out[4]))
+                clock_ghz=out[3], cus=int(out[4]), hbm_write_gbps=out[5],
+                hbm_read_gbps=out[6])
 
 
 class AsmGpuGlobal:
