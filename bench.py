@@ -84,7 +84,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
-              "assembly of one problem runs on a single device in this build.",
+              "assembly of one problem runs on a single device.",
               file=sys.stderr)
         return 2
 
