@@ -21,6 +21,7 @@ reassembler) timed on one host core over a bounded sample.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -40,6 +41,22 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def _stdout_to_stderr():
+    """Route file descriptor 1 to stderr for the duration (the library's
+    test-download call prints its marker line on stdout, which must carry
+    only the result line)."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
 
 
 def cpu_baseline(wl, I, J, seconds: float = 12.0) -> dict:
@@ -217,6 +234,14 @@ def main() -> int:
         k["frac"] = k["achieved"] / k["peak"]
     dom_name = max(kernels, key=lambda k: kernels[k]["ms"])
     dom = kernels[dom_name]
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    # this workload (default sizes and element split only)
+    traffic = None
+    if args.nx == 36 and args.tensor == "on":
+        traffic = _measured_traffic().get(dom_name)
+        for name, t in _measured_traffic().items():
+            if name in kernels:
+                kernels[name]["traffic_bytes"] = t
     out = {
         "metric": METRIC,
         "value": ad.ne / (ms_step * 1e-3),
@@ -235,7 +260,7 @@ def main() -> int:
                    "elements": ad.ne, "dof": ad.height, "nnz": len(J), "nd": ad.nd,
                    "nq_cc": ad.rule_cc.nq, "nq_mass": ad.rule_mass.nq},
         "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
-                     "unit": dom["unit"], "frac": dom["frac"], "traffic": None,
+                     "unit": dom["unit"], "frac": dom["frac"], "traffic": traffic,
                      "kernel": dom_name},
         "detail": {"element_matrices_ms": tl_avg, "scatter_ms": ts_avg,
                    "step_ms_eager": ms_eager, "step_launch": "hip graph" if args.graph == "on"
@@ -260,7 +285,8 @@ def main() -> int:
     if not args.no_cpu:
         cb = cpu_baseline(wl, I, J)
         # the GPU element matrices of the sample match the oracle's
-        Ag = g.download_local()
+        with _stdout_to_stderr():
+            Ag = g.download_local()
         n = cb["_Ae"].shape[0]
         ref = cb["_Ae"]
         err = float(np.max(np.abs(Ag[cb["_e0"]:cb["_e0"] + n] - ref)) / np.max(np.abs(ref)))
@@ -271,6 +297,21 @@ def main() -> int:
     g.close()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _measured_traffic() -> dict:
+    """Per-kernel HBM bytes per launch of the default workload, from the
+    committed counter summary (`traffic_bytes <kernel> <bytes>` lines)."""
+    out = {}
+    path = ROOT / "profiles" / "hbm_traffic_line_grating_p3.md"
+    try:
+        for line in path.read_text().splitlines():
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes":
+                out[f[1]] = float(f[2])
+    except OSError:
+        pass
+    return out
 
 
 def _cpu_model() -> str:

@@ -345,7 +345,7 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
    case 2 * 16 + 3: return launch_mfma<2, 2, 4, 2>(h, P);
    case 3 * 16 + 0: return launch_mfma_pipe<3, 1, 2>(h, P);
    case 3 * 16 + 1: return launch_mfma_pipe<3, 2, 2>(h, P);
-   case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 3>(h, P);
+   case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
    case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
