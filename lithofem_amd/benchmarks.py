@@ -218,10 +218,14 @@ def rhs(log) -> dict:
     def batched():
         b.compute_local()
         b.scatter_eliminate(True)
-        b.assemble_rhs(True)
+        b.assemble_rhs(True, download=False)
 
     batched()
     t_bat = _sync_time(batched)
+    rhs_kernel_ms = b.timings()["batch_ms"]
+    t0 = time.perf_counter()
+    rb = b.download_rhs()
+    t_dl = time.perf_counter() - t0
     b.close()
     # per angle: own handle, matrix + 2 RHS
     singles = []
@@ -236,16 +240,26 @@ def rhs(log) -> dict:
         for g in singles:
             g.compute_local()
             g.scatter_eliminate(True)
-            g.assemble_rhs(True)
+            g.assemble_rhs(True, download=False)
 
     seq()
     t_seq = _sync_time(seq)
+    # angle k's two columns of the batch are rows [k n, (k+1) n) of each
+    # batched column
+    same = True
+    for k, g in enumerate(singles):
+        r1 = g.download_rhs()
+        same = same and np.array_equal(rb[:, k * ad.height:(k + 1) * ad.height], r1)
     for g in singles:
         g.close()
+    log(f"rhs: batched {t_bat * 1e3:.2f} ms (RHS kernels {rhs_kernel_ms:.2f} ms), per angle "
+        f"{t_seq * 1e3:.2f} ms, download {t_dl * 1e3:.1f} ms, bitwise equal {same}")
     return {"metric": "8 angles x 2 polarisations: batched matrices + RHS speedup vs per angle (p=2)",
             "value": t_seq / t_bat, "unit": "x", "higher_is_better": True,
             "detail": {"elements_per_angle": ad.ne, "batched_ms": t_bat * 1e3,
-                       "sequential_ms": t_seq * 1e3}}
+                       "sequential_ms": t_seq * 1e3, "batched_rhs_kernels_ms": rhs_kernel_ms,
+                       "rhs_download_ms": t_dl * 1e3, "rhs_columns": 2 * len(thetas),
+                       "bitwise_equal_to_per_angle": bool(same)}}
 
 
 def adjoint(log, n_xy=8, nz_scale=0.67, n_fd=2) -> dict:

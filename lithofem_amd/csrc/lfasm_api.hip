@@ -110,6 +110,12 @@ struct lfasm_handle {
    int *region_slab = nullptr, *region_src = nullptr;
    std::vector<int> region_src_host;
    double2 *region_deps = nullptr, *rhs = nullptr;
+   // source elements of the incident set (built at the first RHS assembly
+   // after lfasm_set_incident) and the per-element RHS staging
+   int rhs_nsrc = -1;
+   int *rhs_src = nullptr, *rhs_index = nullptr;
+   double2 *rhs_bloc = nullptr;
+   size_t rhs_cap = 0;
    // timing
    hipEvent_t ev[2 * T_N] = {};
    bool ev_used[T_N] = {};
@@ -183,6 +189,8 @@ void release_mesh(lfasm_handle *h)
    h->nconst = h->nquad = 0;
    dfree(h->pml_tab); dfree(h->pid); dfree(h->eps_off);
    dfree(h->waves); dfree(h->region_slab); dfree(h->region_src); dfree(h->region_deps);
+   dfree(h->rhs_src); dfree(h->rhs_index); dfree(h->rhs_bloc);
+   h->rhs_nsrc = -1;
    h->inc_nrhs = 0;
    for (DevRule *r : {&h->rc, &h->rm})
    {
@@ -208,6 +216,7 @@ void release_global(lfasm_handle *h)
    dfree(h->emask);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
    dfree(h->rhs); dfree(h->d_deps);
+   h->rhs_cap = 0;
    h->d_deps_cap = 0;
    h->nparam = h->nvar = 0;
    h->global_built = false;
@@ -581,6 +590,54 @@ hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
                    epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
    hipLaunchKernelGGL(k_local_tensor, grid, dim3(256), 0, h->stream, T, epar);
    return hipGetLastError();
+}
+
+// Device complex array -> host re / im arrays, in chunks of 64 MB through
+// the handle's two pinned buffers: the copy of chunk k overlaps the
+// host-side split of chunk k-1 (threads).  Either output may be null.
+int download_split(lfasm_handle *h, const double2 *dev, size_t n, double *re, double *im)
+{
+   constexpr size_t CH = (size_t)1 << 22;
+   for (int k = 0; k < 2; ++k)
+   {
+      if (!h->pin[k]) { LF_HIP(hipHostMalloc((void **)&h->pin[k], CH * sizeof(double2))); }
+      if (!h->pin_ev[k]) { LF_HIP(hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming)); }
+   }
+   const size_t nch = (n + CH - 1) / CH;
+   const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+   auto split = [&](size_t c) {
+      const double2 *src = h->pin[c & 1];
+      const size_t off = c * CH, len = std::min(CH, n - off);
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < nthr; ++t)
+      {
+         pool.emplace_back([=]() {
+            const size_t a = len * t / nthr, b = len * (t + 1) / nthr;
+            for (size_t k = a; k < b; ++k)
+            {
+               if (re) { re[off + k] = src[k].x; }
+               if (im) { im[off + k] = src[k].y; }
+            }
+         });
+      }
+      for (auto &th : pool) { th.join(); }
+   };
+   for (size_t c = 0; c <= nch; ++c)
+   {
+      if (c < nch)
+      {
+         const size_t off = c * CH, len = std::min(CH, n - off);
+         LF_HIP(hipMemcpyAsync(h->pin[c & 1], dev + off, len * sizeof(double2),
+                               hipMemcpyDeviceToHost, h->stream));
+         LF_HIP(hipEventRecord(h->pin_ev[c & 1], h->stream));
+      }
+      if (c > 0)
+      {
+         LF_HIP(hipEventSynchronize(h->pin_ev[(c - 1) & 1]));
+         split(c - 1);
+      }
+   }
+   return LFASM_OK;
 }
 
 hipError_t ev_begin(lfasm_handle *h, int t)
@@ -1336,50 +1393,7 @@ int lfasm_download_values(lfasm_handle *h, double *re, double *im)
 {
    if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
    LF_HIP(hipSetDevice(h->device));
-   // chunks of 64 MB through two pinned buffers: the copy of chunk k
-   // overlaps the host-side re/im split of chunk k-1 (threads)
-   constexpr size_t CH = (size_t)1 << 22;
-   for (int k = 0; k < 2; ++k)
-   {
-      if (!h->pin[k]) { LF_HIP(hipHostMalloc((void **)&h->pin[k], CH * sizeof(double2))); }
-      if (!h->pin_ev[k]) { LF_HIP(hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming)); }
-   }
-   const size_t nnz = (size_t)h->nnz;
-   const size_t nch = (nnz + CH - 1) / CH;
-   const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-   auto split = [&](size_t c) {
-      const double2 *src = h->pin[c & 1];
-      const size_t off = c * CH, len = std::min(CH, nnz - off);
-      std::vector<std::thread> pool;
-      for (unsigned t = 0; t < nthr; ++t)
-      {
-         pool.emplace_back([=]() {
-            const size_t a = len * t / nthr, b = len * (t + 1) / nthr;
-            for (size_t k = a; k < b; ++k)
-            {
-               if (re) { re[off + k] = src[k].x; }
-               if (im) { im[off + k] = src[k].y; }
-            }
-         });
-      }
-      for (auto &th : pool) { th.join(); }
-   };
-   for (size_t c = 0; c <= nch; ++c)
-   {
-      if (c < nch)
-      {
-         const size_t off = c * CH, len = std::min(CH, nnz - off);
-         LF_HIP(hipMemcpyAsync(h->pin[c & 1], h->vals + off, len * sizeof(double2),
-                               hipMemcpyDeviceToHost, h->stream));
-         LF_HIP(hipEventRecord(h->pin_ev[c & 1], h->stream));
-      }
-      if (c > 0)
-      {
-         LF_HIP(hipEventSynchronize(h->pin_ev[(c - 1) & 1]));
-         split(c - 1);
-      }
-   }
-   return LFASM_OK;
+   return download_split(h, h->vals, (size_t)h->nnz, re, im);
 }
 
 int lfasm_download_local(lfasm_handle *h, double *Ae)
@@ -1712,6 +1726,8 @@ int lfasm_set_incident(lfasm_handle *h, const lfasm_incident *inc)
    h->inc_nrhs = inc->nrhs;
    h->inc_nslab = inc->nslab;
    h->region_src_host.assign(inc->region_src, inc->region_src + h->nreg);
+   dfree(h->rhs_src); dfree(h->rhs_index); dfree(h->rhs_bloc);
+   h->rhs_nsrc = -1;
    return LFASM_OK;
 }
 
@@ -1722,37 +1738,50 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
       return h ? h->fail(LFASM_E_STATE, "build_global and set_incident first") : LFASM_E_STATE;
    }
    LF_HIP(hipSetDevice(h->device));
-   // source elements: those whose region is a source region
-   std::vector<int> attr((size_t)h->ne), pid((size_t)h->ne, 0), eps_off((size_t)h->nmesh);
-   LF_HIP(hipMemcpy(attr.data(), h->attr, attr.size() * sizeof(int), hipMemcpyDeviceToHost));
-   if (h->pid) { LF_HIP(hipMemcpy(pid.data(), h->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost)); }
-   LF_HIP(hipMemcpy(eps_off.data(), h->eps_off, eps_off.size() * sizeof(int), hipMemcpyDeviceToHost));
-   std::vector<int> src, index((size_t)h->ne, -1);
-   for (int e = 0; e < h->ne; ++e)
-   {
-      const int reg = eps_off[(size_t)pid[(size_t)e]] + attr[(size_t)e] - 1;
-      if (h->region_src_host[(size_t)reg]) { index[(size_t)e] = (int)src.size(); src.push_back(e); }
-   }
    const int nrhs = h->inc_nrhs, nd = h->nd;
-   dfree(h->rhs);
-   LF_HIP(hipMalloc((void **)&h->rhs, (size_t)nrhs * h->height * sizeof(double2)));
-   int *d_src = nullptr, *d_index = nullptr;
-   double2 *bloc = nullptr;
-   hipError_t e = upload(h, d_index, index.data(), index.size());
-   if (e == hipSuccess && !src.empty()) { e = upload(h, d_src, src.data(), src.size()); }
-   if (e == hipSuccess && !src.empty())
+   if (h->rhs_nsrc < 0)
    {
-      e = hipMalloc((void **)&bloc, src.size() * nrhs * nd * sizeof(double2));
+      // source elements: those whose region is a source region
+      std::vector<int> attr((size_t)h->ne), pid((size_t)h->ne, 0), eps_off((size_t)h->nmesh);
+      LF_HIP(hipMemcpy(attr.data(), h->attr, attr.size() * sizeof(int), hipMemcpyDeviceToHost));
+      if (h->pid) { LF_HIP(hipMemcpy(pid.data(), h->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost)); }
+      LF_HIP(hipMemcpy(eps_off.data(), h->eps_off, eps_off.size() * sizeof(int), hipMemcpyDeviceToHost));
+      std::vector<int> src, index((size_t)h->ne, -1);
+      for (int e = 0; e < h->ne; ++e)
+      {
+         const int reg = eps_off[(size_t)pid[(size_t)e]] + attr[(size_t)e] - 1;
+         if (h->region_src_host[(size_t)reg]) { index[(size_t)e] = (int)src.size(); src.push_back(e); }
+      }
+      LF_HIP(upload(h, h->rhs_index, index.data(), index.size()));
+      LF_HIP(upload(h, h->rhs_src, src.data(), src.size()));
+      dfree(h->rhs_bloc);
+      if (!src.empty())
+      {
+         LF_HIP(hipMalloc((void **)&h->rhs_bloc, src.size() * nrhs * nd * sizeof(double2)));
+      }
+      h->rhs_nsrc = (int)src.size();
    }
+   const size_t nrhs_entries = (size_t)nrhs * h->height;
+   if (h->rhs_cap != nrhs_entries)
+   {
+      dfree(h->rhs);
+      h->rhs_cap = 0;
+      LF_HIP(hipMalloc((void **)&h->rhs, nrhs_entries * sizeof(double2)));
+      h->rhs_cap = nrhs_entries;
+   }
+   const int nsrc = h->rhs_nsrc;
+   const int *d_src = h->rhs_src, *d_index = h->rhs_index;
+   double2 *bloc = h->rhs_bloc;
+   hipError_t e = hipSuccess;
    if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
-   if (e == hipSuccess && !src.empty())
+   if (e == hipSuccess && nsrc > 0)
    {
       RhsParams R;
       R.nd = nd;
       R.p = h->p;
       R.nrhs = nrhs;
       R.nslab = h->inc_nslab;
-      R.nsrc = (int)src.size();
+      R.nsrc = nsrc;
       R.src_elem = d_src;
       R.attr = h->attr;
       R.pid = h->pid;
@@ -1776,13 +1805,13 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
                               (int)shmem);
       if (e == hipSuccess)
       {
-         hipLaunchKernelGGL(k_rhs_local, dim3((unsigned)src.size()), dim3(64), shmem, h->stream, R);
+         hipLaunchKernelGGL(k_rhs_local, dim3((unsigned)nsrc), dim3(64), shmem, h->stream, R);
          e = hipGetLastError();
       }
    }
    if (e == hipSuccess)
    {
-      if (src.empty())
+      if (nsrc == 0)
       {
          hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, h->stream, h->rhs,
                             (size_t)nrhs * h->height);
@@ -1797,9 +1826,6 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
    }
    if (e == hipSuccess) { e = ev_end(h, T_BATCH); }
    if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
-   if (d_src) { (void)hipFree(d_src); }
-   if (d_index) { (void)hipFree(d_index); }
-   if (bloc) { (void)hipFree(bloc); }
    LF_HIP(e);
    return LFASM_OK;
 }
@@ -1808,15 +1834,7 @@ int lfasm_download_rhs(lfasm_handle *h, double *re, double *im)
 {
    if (!h || !h->rhs) { return h ? h->fail(LFASM_E_STATE, "assemble_rhs first") : LFASM_E_STATE; }
    LF_HIP(hipSetDevice(h->device));
-   std::vector<double2> t((size_t)h->inc_nrhs * h->height);
-   LF_HIP(hipMemcpyAsync(t.data(), h->rhs, t.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
-   LF_HIP(hipStreamSynchronize(h->stream));
-   for (size_t k = 0; k < t.size(); ++k)
-   {
-      if (re) { re[k] = t[k].x; }
-      if (im) { im[k] = t[k].y; }
-   }
-   return LFASM_OK;
+   return download_split(h, h->rhs, (size_t)h->inc_nrhs * h->height, re, im);
 }
 
 int lfasm_eps_sensitivity(lfasm_handle *h, int nparam, const int *elems, const double *lam,

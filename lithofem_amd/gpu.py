@@ -353,8 +353,14 @@ class AsmGpuGlobal:
         self._check(lib().lfasm_set_incident(self._h, ct.byref(st)))
         self.nrhs = inc.nrhs
 
-    def assemble_rhs(self, eliminate=True) -> np.ndarray:
+    def assemble_rhs(self, eliminate=True, download=True):
+        """RHS of every incident field on the device; returns them as an
+        (nrhs, n) complex array when `download`, else None (see
+        download_rhs)."""
         self._check(lib().lfasm_assemble_rhs(self._h, 1 if eliminate else 0))
+        return self.download_rhs() if download else None
+
+    def download_rhs(self) -> np.ndarray:
         re = np.empty(self.nrhs * self.n)
         im = np.empty(self.nrhs * self.n)
         self._check(lib().lfasm_download_rhs(self._h, _p(re), _p(im)))

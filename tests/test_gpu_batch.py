@@ -134,6 +134,15 @@ def test_batched_rhs_matches_oracle(name, p):
         g.build_global(I, J, ad.ess)
         g.set_incident(inc)
         b = g.assemble_rhs(eliminate=True)
+        # the source lists and buffers are kept between calls: a repeat is
+        # bitwise the same, a new incident set (other column count) is
+        # rebuilt, and the original set again reproduces the first result
+        assert np.array_equal(g.assemble_rhs(eliminate=True), b)
+        g.set_incident(incident_set([pb], [fields[1:2]]))
+        b1 = g.assemble_rhs(eliminate=True)
+        assert b1.shape == (1, ad.height) and np.array_equal(b1[0], b[1])
+        g.set_incident(inc)
+        assert np.array_equal(g.assemble_rhs(eliminate=True), b)
     finally:
         g.close()
     orc = O.Oracle(ad, pb, I, J)
