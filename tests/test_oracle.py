@@ -9,12 +9,17 @@ that loses or duplicates contributions) fails at least one of them:
 * closed forms: PML stretch values; lowest-order (Whitney) element
   matrices of all five integrators from barycentric gradients, without and
   with a constant complex PML stretch;
-* invariants: symmetry, CROSS1 = -CROSS0^T, curl-curl null space dimension;
+* invariants: symmetry, CROSS1 = -CROSS0^T, curl-curl null space dimension,
+  the eps-free mass is SPD, generalised eigenvalues of (curl-curl, mass)
+  do not change under the face-DOF transform, sum of detJ / 6 = volume;
 * global identities through face transform, signs and scatter: gradients
   of global polynomials are in the curl-curl kernel; closed-form quadratic
   forms of constant and linear fields;
 * brute force on tiny meshes: CSR pattern, scatter, elimination, residual;
-* finite differences for the eps sensitivity.
+* finite differences for the eps sensitivity;
+* end to end: manufactured solution on the unit cube (curl curl E -
+  k0^2 eps E = f, non-homogeneous PEC), L2 convergence rate >= p - 0.2,
+  complex symmetry and solve residual as in the reference's M4 test.
 """
 
 from __future__ import annotations
@@ -32,6 +37,7 @@ from lithofem_amd.mesh import structured_cell_mesh
 from lithofem_amd.nedelec import EDGES, ND_T, ND_TINV
 from lithofem_amd.problem import Problem, Region, line_grating
 from oracle import oracle as O
+import mms
 
 GOLDEN = Path(__file__).with_name("golden")
 
@@ -153,6 +159,57 @@ def test_element_invariants(p):
     sv = np.linalg.svd(cc, compute_uv=False)
     null = int(np.sum(sv < 1e-10 * sv[0]))
     assert null == (p + 1) * (p + 2) * (p + 3) // 6 - 1
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_mass_spd_and_transform_invariance(p):
+    """eps-free mass of a vacuum element is SPD; the face transform is a
+    congruence, so the generalised eigenvalues of (curl-curl, mass) of a
+    constant-coefficient element are the same with and without it."""
+    import scipy.linalg as sl
+    c, ad, I, J = case("grating", p, 0.0, "tiny")
+    pb = c.problem
+    orc = O.Oracle(ad, pb, I, J)
+    vac = [e for e in range(ad.ne)
+           if not pb.regions[ad.attr[e] - 1].is_pml and pb.regions[ad.attr[e] - 1].eps == 1.0]
+    e = vac[len(vac) // 2]
+    for te in (False, True):
+        m0 = orc.element(e, O.MASS, te) / (-pb.k0 ** 2)
+        assert np.max(np.abs(m0.imag)) == 0.0
+        assert np.linalg.eigvalsh(m0.real).min() > 0.0
+    lam = []
+    for te in (False, True):
+        cc = orc.element(e, O.CURLCURL, te).real
+        m0 = (orc.element(e, O.MASS, te) / (-pb.k0 ** 2)).real
+        lam.append(np.sort(sl.eigvalsh(cc, m0)))
+    assert np.max(np.abs(lam[0] - lam[1])) < 1e-9 * np.max(np.abs(lam[0]))
+    # and the transform does change the matrices themselves (p >= 2)
+    if p >= 2:
+        assert maxnorm_err(orc.element(e, O.CURLCURL, True), orc.element(e, O.CURLCURL, False)) > 1e-3
+
+
+def test_geometry_volume():
+    """Element Jacobians tile the unit cube: sum of detJ / 6 = 1 (interior
+    vertices moved by a smooth map that fixes the boundary)."""
+    ad = build_asm_data(mms.cube_mesh(5), 1)
+    assert np.all(ad.detJ > 0)
+    assert abs(np.sum(ad.detJ) / 6.0 - 1.0) < 1e-13
+
+
+def _orc_assemble(ad, pb, I, J):
+    orc = O.Oracle(ad, pb, I, J)
+    return orc.scatter(orc.elements(0, ad.ne, O.CURLCURL | O.MASS, True))
+
+
+@pytest.mark.parametrize("eps", [2.25, 2.25 + 0.75j], ids=["real", "complex"])
+@pytest.mark.parametrize("p,ns", [(1, (4, 8)), (2, (2, 4)), (3, (2, 4))], ids=["p1", "p2", "p3"])
+def test_mms_convergence(p, ns, eps):
+    rate, levels = mms.convergence(p, eps, ns, _orc_assemble)
+    print(f"p={p} eps={eps}: rate {rate:.3f}, L2 errors {[f'{e:.3e}' for _, e, _, _ in levels]}")
+    assert rate >= p - 0.2, rate
+    for h, err, sym, res in levels:
+        assert sym < 1e-12, sym
+        assert res < 1e-10, res
 
 
 # ---------------------------------------------------------------- global identities
