@@ -238,17 +238,13 @@ __device__ __forceinline__ void piola_v(const volatile double *ec, double a0, do
 // Form one (point, basis k) item of a chunk into the LDS staging rows
 // A[row][k], Cr[row][k], Ci[row][k] (row stride NP).  qc = per-point
 // factors (wa.re, wa.im, wb.re, wb.im).  valid == false writes zero rows.
-__device__ __forceinline__ void form_cc_item(const LocalParams &P, const double *ec,
-                                             const double *qc, int q, int k, bool valid,
-                                             double *A, double *Cr, double *Ci, int row0,
-                                             int NP)
+__device__ __forceinline__ void form_cc_raw(const double *ec, const double *qc, bool valid,
+                                            bool live, double h0, double h1, double h2,
+                                            double *A, double *Cr, double *Ci, int row0, int k,
+                                            int NP)
 {
    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-   if (valid && k < P.nd)
-   {
-      const double *cs = P.rc.cs + ((size_t)q * P.nd + k) * 3;
-      piola_c(ec, cs[0], cs[1], cs[2], c0, c1, c2);
-   }
+   if (live) { piola_c(ec, h0, h1, h2, c0, c1, c2); }
    const double war = valid ? qc[0] : 0.0, wai = valid ? qc[1] : 0.0;
    const double wbr = valid ? qc[2] : 0.0, wbi = valid ? qc[3] : 0.0;
    A[(row0 + 0) * NP + k] = c0;
@@ -262,21 +258,33 @@ __device__ __forceinline__ void form_cc_item(const LocalParams &P, const double 
    Ci[(row0 + 2) * NP + k] = wbi * c2;
 }
 
-__device__ __forceinline__ void form_mass_item(const LocalParams &P, const double *ec,
-                                               const double *qc, int q, int k, bool valid,
-                                               int rows, double *A, double *Cr, double *Ci,
-                                               int row0, int NP)
+__device__ __forceinline__ void form_cc_item(const LocalParams &P, const double *ec,
+                                             const double *qc, int q, int k, bool valid,
+                                             double *A, double *Cr, double *Ci, int row0,
+                                             int NP)
+{
+   const bool live = valid && k < P.nd;
+   double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+   if (live)
+   {
+      const double *cs = P.rc.cs + ((size_t)q * P.nd + k) * 3;
+      h0 = cs[0];
+      h1 = cs[1];
+      h2 = cs[2];
+   }
+   form_cc_raw(ec, qc, valid, live, h0, h1, h2, A, Cr, Ci, row0, k, NP);
+}
+
+__device__ __forceinline__ void form_mass_raw(const double *ec, const double *qc, bool valid,
+                                              bool live, int rows, const double *hv,
+                                              const double *hc, double *A, double *Cr,
+                                              double *Ci, int row0, int k, int NP)
 {
    double v0 = 0.0, v1 = 0.0, v2 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
-   if (valid && k < P.nd)
+   if (live)
    {
-      const double *vs = P.rm.vs + ((size_t)q * P.nd + k) * 3;
-      piola_v(ec, vs[0], vs[1], vs[2], v0, v1, v2);
-      if (rows == 6)
-      {
-         const double *cs = P.rm.cs + ((size_t)q * P.nd + k) * 3;
-         piola_c(ec, cs[0], cs[1], cs[2], c0, c1, c2);
-      }
+      piola_v(ec, hv[0], hv[1], hv[2], v0, v1, v2);
+      if (rows == 6) { piola_c(ec, hc[0], hc[1], hc[2], c0, c1, c2); }
    }
    const double2 wa = valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
    const double2 wb = valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
@@ -309,6 +317,95 @@ __device__ __forceinline__ void form_mass_item(const LocalParams &P, const doubl
       Ci[(row0 + 4) * NP + k] = C4.y;
       Cr[(row0 + 5) * NP + k] = C5.x;
       Ci[(row0 + 5) * NP + k] = C5.y;
+   }
+}
+
+__device__ __forceinline__ void form_mass_item(const LocalParams &P, const double *ec,
+                                               const double *qc, int q, int k, bool valid,
+                                               int rows, double *A, double *Cr, double *Ci,
+                                               int row0, int NP)
+{
+   const bool live = valid && k < P.nd;
+   double hv[3] = {0.0, 0.0, 0.0}, hc[3] = {0.0, 0.0, 0.0};
+   if (live)
+   {
+      const double *vs = P.rm.vs + ((size_t)q * P.nd + k) * 3;
+      hv[0] = vs[0];
+      hv[1] = vs[1];
+      hv[2] = vs[2];
+      if (rows == 6)
+      {
+         const double *cs = P.rm.cs + ((size_t)q * P.nd + k) * 3;
+         hc[0] = cs[0];
+         hc[1] = cs[1];
+         hc[2] = cs[2];
+      }
+   }
+   form_mass_raw(ec, qc, valid, live, rows, hv, hc, A, Cr, Ci, row0, k, NP);
+}
+
+// One item (point, basis k) of chunk `ch` per thread, split into the
+// reference-table loads and the forming, so a kernel can issue the loads
+// of chunk c+1 before the MFMAs of chunk c.  Valid when the chunk has at
+// most one item per thread (NP = 16 NT columns, 64 NT threads).
+struct ChunkItem {
+   double h[6];
+   int q, k, row0;
+   bool has, valid, live, mass;
+};
+
+__device__ __forceinline__ ChunkItem load_chunk_item(const LocalParams &P, const StreamShape &ss,
+                                                     int ch, int tid, int NP)
+{
+   ChunkItem it;
+#pragma unroll
+   for (int i = 0; i < 6; ++i) { it.h[i] = 0.0; }
+   it.mass = ch >= ss.nch_cc;
+   const int qpc = it.mass ? ss.qpc_m : ss.qpc_cc;
+   const int nq = it.mass ? ss.nq_m : ss.nq_cc;
+   const int c = it.mass ? ch - ss.nch_cc : ch;
+   const int rows = it.mass ? ss.rows_m : 3;
+   it.has = tid < qpc * NP;
+   const int qi = tid / NP;
+   it.k = tid - qi * NP;
+   const int q = c * qpc + qi;
+   it.valid = it.has && q < nq;
+   it.q = it.valid ? q : 0;
+   it.row0 = qi * rows;
+   it.live = it.valid && it.k < P.nd;
+   if (it.live)
+   {
+      const RuleDev &r = it.mass ? P.rm : P.rc;
+      const double *t0 = (it.mass ? r.vs : r.cs) + ((size_t)it.q * P.nd + it.k) * 3;
+      it.h[0] = t0[0];
+      it.h[1] = t0[1];
+      it.h[2] = t0[2];
+      if (it.mass && rows == 6)
+      {
+         const double *t1 = r.cs + ((size_t)it.q * P.nd + it.k) * 3;
+         it.h[3] = t1[0];
+         it.h[4] = t1[1];
+         it.h[5] = t1[2];
+      }
+   }
+   return it;
+}
+
+__device__ __forceinline__ void form_chunk_item(const LocalParams &P, const double *ec,
+                                                const StreamShape &ss, const double *qcoef,
+                                                const ChunkItem &it, double *A, double *Cr,
+                                                double *Ci, int NP)
+{
+   if (!it.has) { return; }
+   if (!it.mass)
+   {
+      form_cc_raw(ec, qcoef + (size_t)it.q * 4, it.valid, it.live, it.h[0], it.h[1], it.h[2],
+                  A, Cr, Ci, it.row0, it.k, NP);
+   }
+   else
+   {
+      form_mass_raw(ec, qcoef + (size_t)(P.rc.nq + it.q) * 4, it.valid, it.live, ss.rows_m,
+                    it.h, it.h + 3, A, Cr, Ci, it.row0, it.k, NP);
    }
 }
 
@@ -572,6 +669,10 @@ k_local_mfma_pipe(LocalParams P, int nqt)
       const double *A = buf + (ch & 1) * STG;
       const double *Cr = A + CHUNK_ROWS * NP;
       const double *Ci = Cr + CHUNK_ROWS * NP;
+      // the reference-table loads of the next chunk go out before this
+      // chunk's MFMAs (the last chunk reloads its own item, unused)
+      const bool more = ch + 1 < nch;
+      const ChunkItem nxt = load_chunk_item(P, ss, more ? ch + 1 : ch, gtid, NP);
 #pragma unroll
       for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
       {
@@ -591,11 +692,11 @@ k_local_mfma_pipe(LocalParams P, int nqt)
             acc[0][j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bi[j], acc[0][j][1], 0, 0, 0);
          }
       }
-      if (ch + 1 < nch)
+      if (more)
       {
          double *An = buf + ((ch + 1) & 1) * STG;
-         form_chunk(P, ec, ss, qcoef, ch + 1, gtid, NT * 64, An, An + CHUNK_ROWS * NP,
-                    An + 2 * CHUNK_ROWS * NP, NP);
+         form_chunk_item(P, ec, ss, qcoef, nxt, An, An + CHUNK_ROWS * NP,
+                         An + 2 * CHUNK_ROWS * NP, NP);
       }
       __syncthreads();
    }
