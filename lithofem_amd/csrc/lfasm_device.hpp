@@ -219,7 +219,7 @@ __device__ __forceinline__ void elem_setup(const LocalParams &P, int e, int nqt,
 }
 
 // Contravariant Piola c = (J/detJ) chat, covariant v = J^{-T} vhat.
-__device__ __forceinline__ void piola_c(const volatile double *ec, double h0, double h1, double h2,
+__device__ __forceinline__ void piola_c(const double *ec, double h0, double h1, double h2,
                                         double &c0, double &c1, double &c2)
 {
    c0 = ec[0] * h0 + ec[1] * h1 + ec[2] * h2;
@@ -227,7 +227,7 @@ __device__ __forceinline__ void piola_c(const volatile double *ec, double h0, do
    c2 = ec[6] * h0 + ec[7] * h1 + ec[8] * h2;
 }
 
-__device__ __forceinline__ void piola_v(const volatile double *ec, double a0, double a1, double a2,
+__device__ __forceinline__ void piola_v(const double *ec, double a0, double a1, double a2,
                                         double &v0, double &v1, double &v2)
 {
    v0 = a0 * ec[9] + a1 * ec[12] + a2 * ec[15];
@@ -288,7 +288,7 @@ __device__ __forceinline__ void form_mass_raw(const double *ec, const double *qc
    }
    const double2 wa = valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
    const double2 wb = valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
-   const volatile double *mc = ec + 18;   // m0r m0i m1r m1i m2r m2i kxy c0x c0y c1x c1y
+   const double *mc = ec + 18;   // m0r m0i m1r m1i m2r m2i kxy c0x c0y c1x c1y
    const double2 al0 = make_double2(mc[0] * v0 - mc[6] * v1, mc[1] * v0 + mc[8] * c2);
    const double2 al1 = make_double2(mc[2] * v1 - mc[6] * v0, mc[3] * v1 - mc[7] * c2);
    const double2 al2 = make_double2(mc[4] * v2, mc[5] * v2 - mc[8] * c0 + mc[7] * c1);
