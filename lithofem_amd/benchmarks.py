@@ -171,6 +171,22 @@ def variants(log, nvar=64) -> dict:
     batched()
     t_bat = _sync_time(batched, reps=2)
     t_kernel = g.timings()["batch_ms"]
+    # phases of one batched run (each call returns after its work is done)
+    t0 = time.perf_counter()
+    g.set_eps_elem(None)
+    g.compute_local()
+    g.scatter_eliminate(True)
+    import torch
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g.variants_prepare(params)
+    t2 = time.perf_counter()
+    g.variants_assemble(deps, eliminate=True)
+    t3 = time.perf_counter()
+    phases = {"base_assembly_ms": (t1 - t0) * 1e3, "prepare_ms": (t2 - t1) * 1e3,
+              "upload_and_variant_kernel_ms": (t3 - t2) * 1e3}
+    log("variants: " + ", ".join(f"{k} {v:.2f}" for k, v in phases.items()) +
+        f"; kernel {t_kernel:.2f} ms; batched {t_bat * 1e3:.2f} ms vs sequential {t_seq * 1e3:.2f} ms")
     # spot check: variant 7 against a plain assembly
     vr, vi = g.download_variant(7)
     g.set_eps_elem(ee[7])
@@ -184,7 +200,8 @@ def variants(log, nvar=64) -> dict:
             "detail": {"elements": ad.ne, "dof": ad.height, "nnz": len(J),
                        "params": len(params), "variants": nvar,
                        "sequential_ms": t_seq * 1e3, "batched_ms": t_bat * 1e3,
-                       "variant_kernel_ms": t_kernel, "variant7_maxnorm_vs_plain": err,
+                       "variant_kernel_ms": t_kernel, "phases": phases,
+                       "variant7_maxnorm_vs_plain": err,
                        "output_GB": nvar * len(J) * 16 / 1e9}}
 
 

@@ -1521,6 +1521,41 @@ int upload_pinned(lfasm_handle *h, void *dst, const void *src, size_t bytes)
    return LFASM_OK;
 }
 
+// Unit-eps mass matrices (face-transformed) of the parameter elements:
+// the reference-tensor kernel when every one of them has constant
+// coefficients, else the quadrature kernels.
+int variants_mhat(lfasm_handle *h)
+{
+   const int nparam = h->nparam;
+   LocalParams P = local_params(h, 0, LFASM_MASS, 1, h->Mhat);
+   P.nout = nparam;
+   P.elist = h->pelem;
+   P.eps_unit = 1;
+   bool all_const = h->Rten != nullptr;
+   for (int i = 0; all_const && i < nparam; ++i)
+   {
+      all_const = h->eclass_host[(size_t)h->pelem_host[(size_t)i]] != 0;
+   }
+   if (all_const)
+   {
+      TensorParams T = tensor_params(h, P);
+      T.nlist = nparam;
+      T.elist = h->pelem;
+      T.out_compact = 1;
+      const int epar = T.nb > 128 ? 1 : 256 / T.nb;
+      const dim3 grid((unsigned)((nparam + TEN_EPC - 1) / TEN_EPC),
+                      epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
+      hipLaunchKernelGGL(k_local_tensor, grid, dim3(256), 0, h->stream, T, epar);
+      LF_HIP(hipGetLastError());
+   }
+   else
+   {
+      LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1));
+   }
+   LF_HIP(hipStreamSynchronize(h->stream));
+   return LFASM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1537,8 +1572,14 @@ int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems)
       }
    }
    LF_HIP(hipSetDevice(h->device));
-   dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat);
    const size_t nd = (size_t)h->nd;
+   // the same parameter set again: its contributor lists and buffers stand,
+   // only the unit-eps matrices are formed anew
+   const bool same = h->nparam == nparam && h->Mhat &&
+                     std::equal(elems, elems + nparam, h->pelem_host.begin());
+   if (same) { return variants_mhat(h); }
+   dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat);
+   h->nparam = 0;
    h->pelem_host.assign(elems, elems + nparam);
    LF_HIP(upload(h, h->pelem, elems, (size_t)nparam));
    // per-row parameter contributors, ascending element order
@@ -1560,34 +1601,8 @@ int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems)
    LF_HIP(upload(h, h->prc_ptr, ptr.data(), ptr.size()));
    LF_HIP(upload(h, h->prc_list, lst.data(), lst.size()));
    LF_HIP(hipMalloc((void **)&h->Mhat, (size_t)nparam * nd * nd * sizeof(double2)));
-   // unit-eps mass matrices (face-transformed) of the parameter elements:
-   // the reference-tensor kernel when every one of them has constant
-   // coefficients, else the quadrature kernels
-   LocalParams P = local_params(h, 0, LFASM_MASS, 1, h->Mhat);
-   P.nout = nparam;
-   P.elist = h->pelem;
-   P.eps_unit = 1;
-   bool all_const = h->Rten != nullptr;
-   for (int i = 0; all_const && i < nparam; ++i) { all_const = h->eclass_host[(size_t)elems[i]] != 0; }
-   if (all_const)
-   {
-      TensorParams T = tensor_params(h, P);
-      T.nlist = nparam;
-      T.elist = h->pelem;
-      T.out_compact = 1;
-      const int epar = T.nb > 128 ? 1 : 256 / T.nb;
-      const dim3 grid((unsigned)((nparam + TEN_EPC - 1) / TEN_EPC),
-                      epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
-      hipLaunchKernelGGL(k_local_tensor, grid, dim3(256), 0, h->stream, T, epar);
-      LF_HIP(hipGetLastError());
-   }
-   else
-   {
-      LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1));
-   }
-   LF_HIP(hipStreamSynchronize(h->stream));
    h->nparam = nparam;
-   return LFASM_OK;
+   return variants_mhat(h);
 }
 
 int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int eliminate)
