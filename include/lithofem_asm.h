@@ -233,7 +233,9 @@ int lfasm_timings(const lfasm_handle *h, double *ms, int n);
  * Many per-element eps sets on one mesh by delta assembly: the mass term is
  * linear in eps, so A(eps_v) = A_base + sum_e deps_{v,e} Mhat_e with Mhat_e
  * the face-transformed unit-eps mass matrix of parameter element e.
- * prepare: parameter elements (sorted, unique element ids); computes Mhat.
+ * prepare: parameter elements (sorted, unique element ids); computes Mhat
+ * (called again with the same elements it keeps the per-row parameter
+ * contributor lists and forms only Mhat anew).
  * assemble: A_base = the handle's current CSR values (compute_local +
  * scatter_eliminate with the same `eliminate`); deps = nvar x nparam
  * interleaved complex; writes nvar value arrays on the device. */
