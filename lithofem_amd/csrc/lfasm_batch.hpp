@@ -59,9 +59,16 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
    const bool row_fixed = c0 == c1 || (V.eliminate && V.is_ess[r]);
    if (row_fixed)
    {
-      for (int v = 0; v < V.nvar; ++v)
+      // the base row, 64 entries at a time, held in registers across the
+      // variants (the stores may alias the loads for the compiler)
+      for (int s0 = 0; s0 < len; s0 += 64)
       {
-         for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = V.base[base + s]; }
+         const int s = s0 + lane;
+         const double2 x = s < len ? V.base[base + s] : make_double2(0.0, 0.0);
+         if (s < len)
+         {
+            for (int v = 0; v < V.nvar; ++v) { V.out[(size_t)v * V.nnz + base + s] = x; }
+         }
       }
       return;
    }
@@ -142,9 +149,16 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
    const bool row_fixed = c0 == c1 || (V.eliminate && V.is_ess[r]);
    if (row_fixed)
    {
-      for (int v = 0; v < V.nvar; ++v)
+      // the base row, 64 entries at a time, held in registers across the
+      // variants (the stores may alias the loads for the compiler)
+      for (int s0 = 0; s0 < len; s0 += 64)
       {
-         for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = V.base[base + s]; }
+         const int s = s0 + lane;
+         const double2 x = s < len ? V.base[base + s] : make_double2(0.0, 0.0);
+         if (s < len)
+         {
+            for (int v = 0; v < V.nvar; ++v) { V.out[(size_t)v * V.nnz + base + s] = x; }
+         }
       }
       return;
    }
