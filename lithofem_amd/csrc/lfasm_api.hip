@@ -2055,6 +2055,34 @@ int lfasm_measure_peaks(int device, double *out)
       if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
       out[6] = (double)nbytes / (ms * 1e-3) / 1e9;
    }
+   // launch-shape and store-mode variants: keep the best of each stream
+   {
+      const pk_dv2 *va = reinterpret_cast<const pk_dv2 *>(a);
+      pk_dv2 *vb = reinterpret_cast<pk_dv2 *>(b);
+      auto timed = [&](auto launch, double bytes, double &best) -> bool {
+         launch();
+         if (!ok(hipEventRecord(e0, s))) { return false; }
+         launch();
+         if (!ok(hipEventRecord(e1, s)) || !ok(hipEventSynchronize(e1)) ||
+             !ok(hipEventElapsedTime(&ms, e0, e1))) { return false; }
+         best = std::max(best, bytes / (ms * 1e-3) / 1e9);
+         return true;
+      };
+      for (int g : {4, 16, 64})
+      {
+         const dim3 grid((unsigned)(ncu * g));
+         const double nb = (double)nbytes;
+         if (!timed([&] { hipLaunchKernelGGL((k_peak_copy_v<1, false>), grid, dim3(256), 0, s, va, vb, n2); }, 2 * nb, out[2]) ||
+             !timed([&] { hipLaunchKernelGGL((k_peak_copy_v<4, true>), grid, dim3(256), 0, s, va, vb, n2); }, 2 * nb, out[2]) ||
+             !timed([&] { hipLaunchKernelGGL((k_peak_write_v<4, false>), grid, dim3(256), 0, s, vb, n2); }, nb, out[5]) ||
+             !timed([&] { hipLaunchKernelGGL((k_peak_write_v<4, true>), grid, dim3(256), 0, s, vb, n2); }, nb, out[5]) ||
+             !timed([&] { hipLaunchKernelGGL((k_peak_read_v<4, false>), grid, dim3(256), 0, s, va, n2, dout); }, nb, out[6]) ||
+             !timed([&] { hipLaunchKernelGGL((k_peak_read_v<4, true>), grid, dim3(256), 0, s, va, n2, dout); }, nb, out[6]))
+         {
+            goto done;
+         }
+      }
+   }
 done:
    if (a) { (void)hipFree(a); }
    if (b) { (void)hipFree(b); }

@@ -92,4 +92,80 @@ __global__ void __launch_bounds__(256) k_peak_read(const double2 *__restrict__ s
    if (acc == 12345.678) { out[0] = acc; }   // keeps the loads; never true for zeros
 }
 
+// Launch-shape and store-mode variants of the streams above (U accesses
+// per thread per iteration, ordinary or non-temporal): the ceiling taken is
+// the best of them, since which pattern reaches the most depends on the
+// access (measured on MI355X: copy best with one access per thread and a
+// 1,024-block grid, writes and reads best non-temporal).
+typedef double pk_dv2 __attribute__((ext_vector_type(2)));
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_peak_copy_v(const pk_dv2 *__restrict__ s,
+                                                     pk_dv2 *__restrict__ d, size_t n)
+{
+   const size_t st = (size_t)gridDim.x * blockDim.x;
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * st)
+   {
+      pk_dv2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+      {
+         const size_t j = i + u * st;
+         v[u] = j < n ? (NT ? __builtin_nontemporal_load(s + j) : s[j]) : pk_dv2{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+      {
+         const size_t j = i + u * st;
+         if (j < n)
+         {
+            if (NT) { __builtin_nontemporal_store(v[u], d + j); }
+            else { d[j] = v[u]; }
+         }
+      }
+   }
+}
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_peak_write_v(pk_dv2 *__restrict__ d, size_t n)
+{
+   const size_t st = (size_t)gridDim.x * blockDim.x;
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * st)
+   {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+      {
+         const size_t j = i + u * st;
+         if (j < n)
+         {
+            const pk_dv2 v = {(double)j, 0.0};
+            if (NT) { __builtin_nontemporal_store(v, d + j); }
+            else { d[j] = v; }
+         }
+      }
+   }
+}
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_peak_read_v(const pk_dv2 *__restrict__ s, size_t n,
+                                                     double *__restrict__ out)
+{
+   const size_t st = (size_t)gridDim.x * blockDim.x;
+   double acc = 0.0;
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * st)
+   {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+      {
+         const size_t j = i + u * st;
+         if (j < n)
+         {
+            const pk_dv2 v = NT ? __builtin_nontemporal_load(s + j) : s[j];
+            acc += v.x + v.y;
+         }
+      }
+   }
+   if (acc == 12345.678) { out[0] = acc; }
+}
+
 }  // namespace lfasm
