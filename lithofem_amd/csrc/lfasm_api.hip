@@ -67,6 +67,7 @@ struct lfasm_handle {
    double TInv[24];
    double *TInv_dev = nullptr;
    double2 *Ae = nullptr;
+   int lda = 0;                       // row stride of the element matrices in Ae
    // constant-coefficient elements (every vertex between z_min and z_max of
    // the element's problem) take the reference-tensor kernel; the others the
    // quadrature kernels
@@ -276,6 +277,7 @@ LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_
    P.mflags = mf;
    P.apply_te = apply_te && h->p >= 2;
    P.Ae = Ae;
+   P.lda = h->lda;
    P.nout = h->ne;
    P.elist = nullptr;
    P.eps_unit = 0;
@@ -313,8 +315,8 @@ hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
                       P, nqt, NP);
    e = hipGetLastError();
    if (e != hipSuccess || !P.apply_te) { return e; }
-   hipLaunchKernelGGL(k_te, dim3(P.nout), dim3(64), 0, h->stream, P.nd, P.p, P.fo, P.elist,
-                      P.out_by_elem, h->TInv_dev, P.Ae);
+   hipLaunchKernelGGL(k_te, dim3(P.nout), dim3(64), 0, h->stream, P.nd, P.lda, P.p, P.fo,
+                      P.elist, P.out_by_elem, h->TInv_dev, P.Ae);
    return hipGetLastError();
 }
 
@@ -534,6 +536,7 @@ hipError_t launch_fused_rows(lfasm_handle *h, int eliminate)
       F.eflag = h->eflag;
       std::memcpy(F.TInv, h->TInv, sizeof(F.TInv));
       F.Ae = h->Ae;
+      F.lda = h->lda;
       F.R = h->Rten;
       F.vals = h->vals;
       const int nw = h->fwaves[t];
@@ -579,12 +582,21 @@ TensorParams tensor_params(lfasm_handle *h, const LocalParams &P)
    T.eps_unit = P.eps_unit;
    T.out_compact = 0;
    T.Ae = P.Ae;
+   T.lda = P.lda;
    return T;
 }
 
 hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
 {
    const TensorParams T = tensor_params(h, P);
+   if (T.lda % 8 == 0 && T.lda > T.nd && !T.out_compact)
+   {
+      // padded rows: the column-per-lane kernel with whole-line stores
+      const dim3 grid((unsigned)((h->nconst + TEN21_EPC - 1) / TEN21_EPC),
+                      (unsigned)((T.npair * T.lda + 255) / 256));
+      hipLaunchKernelGGL(k_local_tensor21, grid, dim3(256), 0, h->stream, T);
+      return hipGetLastError();
+   }
    const int epar = T.nb > 128 ? 1 : 256 / T.nb;
    const dim3 grid((unsigned)((h->nconst + TEN_EPC - 1) / TEN_EPC),
                    epar > 1 ? 1u : (unsigned)((T.nb + 255) / 256));
@@ -640,6 +652,53 @@ int download_split(lfasm_handle *h, const double2 *dev, size_t n, double *re, do
    return LFASM_OK;
 }
 
+// Device rows of `lda` complex entries -> host rows of `nd` (the padding
+// dropped), in chunks through the handle's two pinned 64 MB buffers: the
+// copy of chunk k overlaps the host-side repacking of chunk k-1 (threads).
+int download_unpad(lfasm_handle *h, const double2 *dev, size_t rows, int nd, int lda, double *out)
+{
+   constexpr size_t CH = (size_t)1 << 22;            // double2 entries per buffer
+   for (int k = 0; k < 2; ++k)
+   {
+      if (!h->pin[k]) { LF_HIP(hipHostMalloc((void **)&h->pin[k], CH * sizeof(double2))); }
+      if (!h->pin_ev[k]) { LF_HIP(hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming)); }
+   }
+   const size_t rch = std::max<size_t>(1, CH / (size_t)lda);   // rows per chunk
+   const size_t nch = (rows + rch - 1) / rch;
+   const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+   auto repack = [&](size_t c) {
+      const double2 *src = h->pin[c & 1];
+      const size_t r0 = c * rch, nr = std::min(rch, rows - r0);
+      std::vector<std::thread> pool;
+      for (unsigned t = 0; t < nthr; ++t)
+      {
+         pool.emplace_back([=]() {
+            for (size_t r = nr * t / nthr; r < nr * (t + 1) / nthr; ++r)
+            {
+               std::memcpy(out + 2 * (r0 + r) * nd, src + r * lda, (size_t)nd * sizeof(double2));
+            }
+         });
+      }
+      for (auto &th : pool) { th.join(); }
+   };
+   for (size_t c = 0; c <= nch; ++c)
+   {
+      if (c < nch)
+      {
+         const size_t r0 = c * rch, nr = std::min(rch, rows - r0);
+         LF_HIP(hipMemcpyAsync(h->pin[c & 1], dev + r0 * lda, nr * lda * sizeof(double2),
+                               hipMemcpyDeviceToHost, h->stream));
+         LF_HIP(hipEventRecord(h->pin_ev[c & 1], h->stream));
+      }
+      if (c > 0)
+      {
+         LF_HIP(hipEventSynchronize(h->pin_ev[(c - 1) & 1]));
+         repack(c - 1);
+      }
+   }
+   return LFASM_OK;
+}
+
 hipError_t ev_begin(lfasm_handle *h, int t)
 {
    h->ev_used[t] = true;
@@ -674,8 +733,8 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
    }
    if (!P.use_cc && !P.mflags)
    {
-      const size_t n = (size_t)h->ne * h->nd * h->nd;
-      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, h->Ae, n);
+      const size_t n = (size_t)h->ne * h->nd * h->lda;
+      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, P.Ae, n);
       LF_TRY(hipGetLastError());
    }
    else if (split)
@@ -726,6 +785,7 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.sld = h->sld;
    S.is_ess = h->ess;
    S.Ae = h->Ae;
+   S.lda = h->lda;
    S.vals = h->vals;
    S.eliminate = eliminate ? 1 : 0;
    S.maxrow = h->maxrow;
@@ -1001,7 +1061,9 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
    LF_HIP(upload(h, h->pml_tab, pml.data(), pml.size()));
    std::memcpy(h->TInv, m0.TInv, sizeof(h->TInv));
    LF_HIP(upload(h, h->TInv_dev, h->TInv, 24));
-   LF_HIP(hipMalloc((void **)&h->Ae, ne * nd * nd * sizeof(double2)));
+   // element-matrix rows padded to whole 128-byte lines (8 entries)
+   h->lda = (int)((nd + 7) / 8 * 8);
+   LF_HIP(hipMalloc((void **)&h->Ae, ne * nd * (size_t)h->lda * sizeof(double2)));
    // element classes: s(z) = 1 on the closed slab [z_min, z_max], so an
    // element whose four vertices lie in it has Lambda = I at every
    // quadrature point (the points are convex combinations of the vertices)
@@ -1407,10 +1469,7 @@ int lfasm_download_local(lfasm_handle *h, double *Ae)
       // the fused path never wrote the constant-coefficient elements' Ae
       LF_HIP(launch_tensor(h, h->last_P));
    }
-   LF_HIP(hipMemcpyAsync(Ae, h->Ae, (size_t)h->ne * h->nd * h->nd * sizeof(double2),
-                         hipMemcpyDeviceToHost, h->stream));
-   LF_HIP(hipStreamSynchronize(h->stream));
-   return LFASM_OK;
+   return download_unpad(h, h->Ae, (size_t)h->ne * h->nd, h->nd, h->lda, Ae);
 }
 
 int lfasm_residual(lfasm_handle *h, const double *X2n, const double *B2n, double *rel)
@@ -1528,6 +1587,7 @@ int variants_mhat(lfasm_handle *h)
 {
    const int nparam = h->nparam;
    LocalParams P = local_params(h, 0, LFASM_MASS, 1, h->Mhat);
+   P.lda = h->nd;                     // Mhat rows are unpadded
    P.nout = nparam;
    P.elist = h->pelem;
    P.eps_unit = 1;

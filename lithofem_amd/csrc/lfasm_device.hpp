@@ -51,6 +51,7 @@ struct LocalParams {
    const int *elist;     // output i -> element elist[i] (NULL: element i)
    int eps_unit;         // 1: eps = 1 in the mass term (unit-eps mass matrices)
    int out_by_elem;      // 1: write element elist[i]'s matrix to Ae[elist[i]], not Ae[i]
+   int lda;              // row stride of the element matrices in Ae (>= nd)
 };
 
 // ---- complex helpers ------------------------------------------------------
@@ -526,7 +527,11 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
    }
 
    if (!live) { return; }
-   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * nd;
+   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * P.lda;
+   // rows padded to whole 128-byte lines: the 16 columns of a tile row are
+   // two whole lines (pad columns carry the zero accumulators), so the
+   // stores go out non-temporal; unpadded rows keep ordinary stores
+   const bool lines = (P.lda & 7) == 0;
 #pragma unroll
    for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -537,9 +542,17 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
          for (int v = 0; v < 4; ++v)
          {
             const int row = 16 * (wsub * RT + t) + g4 + 4 * v;
-            if (row < nd && col < nd)
+            if (row < nd && col < P.lda)
             {
-               Ae[(size_t)row * nd + col] = make_double2(acc[t][j][0][v], acc[t][j][1][v]);
+               if (lines)
+               {
+                  __builtin_nontemporal_store(dbl2v{acc[t][j][0][v], acc[t][j][1][v]},
+                                              reinterpret_cast<dbl2v *>(Ae) + (size_t)row * P.lda + col);
+               }
+               else if (col < nd)
+               {
+                  Ae[(size_t)row * P.lda + col] = make_double2(acc[t][j][0][v], acc[t][j][1][v]);
+               }
             }
          }
       }
@@ -756,23 +769,28 @@ k_local_generic(LocalParams P, int nqt, int NP)
       }
       __syncthreads();
    }
-   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * n2;
+   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * P.lda;
 #pragma unroll
    for (int i = 0; i < NACC; ++i)
    {
       const int idx = tid + NTHR * i;
-      if (idx < n2) { Ae[idx] = make_double2(ar[i], ai[i]); }
+      if (idx < n2)
+      {
+         const int k = idx / nd, l = idx - k * nd;
+         Ae[(size_t)k * P.lda + l] = make_double2(ar[i], ai[i]);
+      }
    }
 }
 
 // Dual face-DOF transform in place on Ae (solver/asm_gpu.inc:221-251):
 // one block per element, columns then rows.
-__global__ void k_te(int nd, int p, const int *__restrict__ fo, const int *__restrict__ elist,
-                     int out_by_elem, const double *TInvG, double2 *__restrict__ Ae)
+__global__ void k_te(int nd, int lda, int p, const int *__restrict__ fo,
+                     const int *__restrict__ elist, int out_by_elem, const double *TInvG,
+                     double2 *__restrict__ Ae)
 {
    const int io = blockIdx.x;
    const int e = elist ? elist[io] : io;
-   double2 *A = Ae + (size_t)(out_by_elem ? e : io) * nd * nd;
+   double2 *A = Ae + (size_t)(out_by_elem ? e : io) * nd * lda;
    const int foff = 6 * p;
    const int ntd = p * (p - 1);
    for (int axis = 0; axis < 2; ++axis)
@@ -785,8 +803,8 @@ __global__ void k_te(int nd, int p, const int *__restrict__ fo, const int *__res
             const int d0 = foff + f * ntd + 2 * i;
             for (int r = threadIdx.x; r < nd; r += blockDim.x)
             {
-               double2 &a = axis == 0 ? A[(size_t)r * nd + d0] : A[(size_t)d0 * nd + r];
-               double2 &b = axis == 0 ? A[(size_t)r * nd + d0 + 1] : A[(size_t)(d0 + 1) * nd + r];
+               double2 &a = axis == 0 ? A[(size_t)r * lda + d0] : A[(size_t)d0 * lda + r];
+               double2 &b = axis == 0 ? A[(size_t)r * lda + d0 + 1] : A[(size_t)(d0 + 1) * lda + r];
                const double2 na = make_double2(Ti[0] * a.x + Ti[1] * b.x, Ti[0] * a.y + Ti[1] * b.y);
                const double2 nb = make_double2(Ti[2] * a.x + Ti[3] * b.x, Ti[2] * a.y + Ti[3] * b.y);
                a = na;
@@ -846,6 +864,7 @@ struct ScatterParams {
    const uint16_t *slot;
    const unsigned char *is_ess;
    const double2 *Ae;
+   int lda;                     // row stride of the element matrices in Ae
    double2 *vals;
    int eliminate;
    int maxrow;
@@ -917,7 +936,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
                   use[u] = (fb & ess_bit) == 0;
                   sl[u] = S.slot[(size_t)ea * S.sld + b];
                   const dbl2v x = __builtin_nontemporal_load(
-                     reinterpret_cast<const dbl2v *>(S.Ae) + (size_t)ea * nd + b);
+                     reinterpret_cast<const dbl2v *>(S.Ae) + (size_t)ea * S.lda + b);
                   const bool neg = ((fa ^ fb) & 1) != 0;
                   v[u] = neg ? make_double2(-x.x, -x.y) : make_double2(x.x, x.y);
                }

@@ -103,6 +103,7 @@ struct TensorParams {
    int eps_unit;            // 1: eps = 1 in the mass term (unit-eps mass matrices)
    int out_compact;         // 1: list entry i writes Ae[i] (else Ae[elist[i]])
    double2 *Ae;
+   int lda;                 // row stride of the element matrices in Ae (>= nd)
 };
 
 // Element coefficients (27 doubles) of element e into c[0..27].
@@ -267,13 +268,122 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
             im[2 + col] = pair_hi(Ti, ai, bi);
          }
       }
-      double2 *A = P.Ae + (size_t)sel[i] * nd * nd;
+      double2 *A = P.Ae + (size_t)sel[i] * nd * P.lda;
 #pragma unroll
       for (int s = 0; s < 4; ++s)
       {
          const int k = k0 + (s >> 1), l = l0 + (s & 1);
-         if (k < nd && l < nd) { A[(size_t)k * nd + l] = make_double2(re[s], im[s]); }
+         if (k < nd && l < nd) { A[(size_t)k * P.lda + l] = make_double2(re[s], im[s]); }
       }
+   }
+}
+
+// Column-per-lane form for padded element matrices (row stride lda a
+// multiple of 8 entries, 128-byte aligned rows): a thread owns rows 2rp and
+// 2rp+1 at one column l < lda (pad columns are written as zeros), so the
+// lanes of a wave cover whole row segments and every store instruction
+// writes whole 128-byte lines -- which lets the stores be non-temporal.
+// The column face pair (l, l^1) sits in lanes (t, t^1).  Every entry takes
+// exactly the FMAs of k_local_tensor, so both give bit-identical values.
+typedef double ten_dv2 __attribute__((ext_vector_type(2)));
+constexpr int TEN21_EPC = 64;   // elements per workgroup
+
+__global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P)
+{
+   __shared__ double coef[TEN21_EPC * TEN_NC];
+   __shared__ int sfo[TEN21_EPC * 4];
+   __shared__ int sel[TEN21_EPC];
+   const int tid = threadIdx.x;
+   const int c0 = blockIdx.x * TEN21_EPC;
+   const int cnt = min(TEN21_EPC, P.nlist - c0);
+   if (tid < cnt)
+   {
+      const int e = P.elist[c0 + tid];
+      sel[tid] = P.out_compact ? c0 + tid : e;
+      tensor_coef(P, e, coef + tid * TEN_NC);
+      if (P.p >= 2)
+      {
+#pragma unroll
+         for (int f = 0; f < 4; ++f) { sfo[tid * 4 + f] = P.fo[(size_t)e * 4 + f]; }
+      }
+   }
+   __syncthreads();
+   const int nd = P.nd, lda = P.lda;
+   const int blk = blockIdx.y * 256 + tid;
+   const bool live = blk < P.npair * lda;
+   const int bb = live ? blk : 0;
+   const int rp = bb / lda, l = bb - rp * lda;
+   const int k0 = 2 * rp, odd = l & 1;
+   const bool col_ok = l < nd;
+   // blocked reference tensors: 2x2 block (rp, l/2), entries (k0 + i, l)
+   const double *Rb = P.R + ((size_t)rp * P.npair + (col_ok ? l >> 1 : 0)) * NTENS * 4;
+   double r[NTENS][2];
+#pragma unroll
+   for (int t = 0; t < NTENS; ++t)
+   {
+      r[t][0] = col_ok ? Rb[t * 4 + odd] : 0.0;
+      r[t][1] = col_ok ? Rb[t * 4 + 2 + odd] : 0.0;
+   }
+   const int fr = P.apply_te ? pair_face(k0, P.p) : -1;
+   const int fc = (P.apply_te && col_ok) ? pair_face(l - odd, P.p) : -1;
+   const bool st0 = live && k0 < nd, st1 = live && k0 + 1 < nd;
+
+   for (int i = 0; i < cnt; ++i)
+   {
+      const double *c = coef + i * TEN_NC;
+      double re[2], im[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+      {
+         double x = 0.0, y = 0.0;
+#pragma unroll
+         for (int t = 0; t < 6; ++t)
+         {
+            x = __builtin_fma(c[t], r[t][s], x);
+            x = __builtin_fma(c[6 + 2 * t], r[6 + t][s], x);
+            y = __builtin_fma(c[7 + 2 * t], r[6 + t][s], y);
+         }
+#pragma unroll
+         for (int t = 0; t < 9; ++t) { y = __builtin_fma(c[18 + t], r[12 + t][s], y); }
+         re[s] = x;
+         im[s] = y;
+      }
+      // dual face transform: columns (partner column in lane ^ 1; every
+      // lane takes part in the exchange), then rows (in the thread)
+      const int foc = fc >= 0 ? sfo[i * 4 + fc] : 0;
+      const double *Tc = P.TInv + 4 * foc;
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+      {
+         const double ur = __shfl_xor(re[s], 1), ui = __shfl_xor(im[s], 1);
+         if (fc >= 0)
+         {
+            if (!odd)
+            {
+               re[s] = pair_lo(Tc, re[s], ur);
+               im[s] = pair_lo(Tc, im[s], ui);
+            }
+            else
+            {
+               re[s] = pair_hi(Tc, ur, re[s]);
+               im[s] = pair_hi(Tc, ui, im[s]);
+            }
+         }
+      }
+      if (fr >= 0)
+      {
+         const double *Ti = P.TInv + 4 * sfo[i * 4 + fr];
+         const double ar = re[0], ai = im[0], br = re[1], bi = im[1];
+         re[0] = pair_lo(Ti, ar, br);
+         im[0] = pair_lo(Ti, ai, bi);
+         re[1] = pair_hi(Ti, ar, br);
+         im[1] = pair_hi(Ti, ai, bi);
+      }
+      ten_dv2 *A = reinterpret_cast<ten_dv2 *>(P.Ae) + (size_t)sel[i] * nd * lda;
+      const ten_dv2 v0 = col_ok ? ten_dv2{re[0], im[0]} : ten_dv2{0.0, 0.0};
+      const ten_dv2 v1 = col_ok ? ten_dv2{re[1], im[1]} : ten_dv2{0.0, 0.0};
+      if (st0) { __builtin_nontemporal_store(v0, A + (size_t)k0 * lda + l); }
+      if (st1) { __builtin_nontemporal_store(v1, A + (size_t)(k0 + 1) * lda + l); }
    }
 }
 
@@ -354,6 +464,7 @@ struct FusedParams {
    const unsigned char *eflag;
    double TInv[24];
    const double2 *Ae;
+   int lda;                    // row stride of the element matrices in Ae
    const double *R;            // blocked reference tensors (k_ref_tensor)
    double2 *vals;
 };
@@ -503,12 +614,12 @@ __global__ void __launch_bounds__(768) k_fused_rows(FusedParams F,
             if (enc[k] < 0)
             {
                const dbl2v x = __builtin_nontemporal_load(
-                  reinterpret_cast<const dbl2v *>(F.Ae) + (size_t)ea * nd + bb);
+                  reinterpret_cast<const dbl2v *>(F.Ae) + (size_t)ea * F.lda + bb);
                ap0[k] = make_double2(x.x, x.y);
                if (pair)
                {
                   const dbl2v y = __builtin_nontemporal_load(
-                     reinterpret_cast<const dbl2v *>(F.Ae) + (size_t)(ea + 1) * nd + bb);
+                     reinterpret_cast<const dbl2v *>(F.Ae) + (size_t)(ea + 1) * F.lda + bb);
                   ap1[k] = make_double2(y.x, y.y);
                }
             }
