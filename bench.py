@@ -221,7 +221,7 @@ def main() -> int:
     }
     if nconst > 0:
         q_t = tensor_bytes(nconst, ad.nd)
-        kernels["k_local_tensor"] = {"ms": tt_avg, "bound": "hbm", "unit": "GB/s",
+        kernels["k_local_tensor21"] = {"ms": tt_avg, "bound": "hbm", "unit": "GB/s",
                                      "achieved": q_t / (tt_avg * 1e-3) / 1e9,
                                      "peak": HBM_PEAK_GBS, "bytes": q_t, "elements": nconst}
     if nquad > 0:
