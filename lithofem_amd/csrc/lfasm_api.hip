@@ -104,6 +104,7 @@ struct lfasm_handle {
    std::vector<int> dof_host;         // element dof table (host copy)
    int pmaxc = 0;                     // most parameter contributors of one row
    double2 *d_deps = nullptr;         // eps offsets of the variants (device)
+   double2 *d_depsT = nullptr;        // the same, parameter-major
    size_t d_deps_cap = 0;
    // incident fields / RHS
    int inc_nrhs = 0, inc_nslab = 0;
@@ -216,7 +217,7 @@ void release_global(lfasm_handle *h)
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    dfree(h->emask);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
-   dfree(h->rhs); dfree(h->d_deps);
+   dfree(h->rhs); dfree(h->d_deps); dfree(h->d_depsT);
    h->rhs_cap = 0;
    h->d_deps_cap = 0;
    h->nparam = h->nvar = 0;
@@ -1680,7 +1681,10 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    if (ndeps > h->d_deps_cap)
    {
       dfree(h->d_deps);
+      dfree(h->d_depsT);
+      h->d_deps_cap = 0;
       LF_HIP(hipMalloc((void **)&h->d_deps, ndeps * sizeof(double2)));
+      LF_HIP(hipMalloc((void **)&h->d_depsT, ndeps * sizeof(double2)));
       h->d_deps_cap = ndeps;
    }
    {
@@ -1704,6 +1708,7 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    V.is_ess = h->ess;
    V.Mhat = h->Mhat;
    V.deps = d_deps;
+   V.depsT = h->d_depsT;
    V.base = h->vals;
    V.out = h->var_vals;
    V.eliminate = eliminate ? 1 : 0;
@@ -1717,6 +1722,13 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    }
    if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
+   if (e == hipSuccess && h->nd <= 64 && h->pmaxc <= VMAXC)
+   {
+      const dim3 tg((unsigned)((h->nparam + 31) / 32), (unsigned)((nvar + 31) / 32));
+      hipLaunchKernelGGL(k_transpose_deps, tg, dim3(256), 0, h->stream, d_deps, h->d_depsT, nvar,
+                         h->nparam);
+      e = hipGetLastError();
+   }
    if (e == hipSuccess)
    {
       if (h->nd <= 64 && h->pmaxc <= VMAXC)

@@ -32,6 +32,7 @@ struct VariantParams {
    const unsigned char *is_ess;
    const double2 *Mhat;          // nparam x nd x nd
    const double2 *deps;          // nvar x nparam
+   const double2 *depsT;         // nparam x nvar (the same values, parameter-major)
    const double2 *base;          // nnz
    double2 *out;                 // nvar x nnz
    int eliminate, maxrow;
@@ -43,6 +44,36 @@ struct VariantParams {
 // Mhat rows are read once per row instead of once per variant; the
 // variants then only rescale and accumulate.  Same sums, same order.
 constexpr int VMAXC = 16;
+
+// value of lane `src` (wave-uniform) of a double held one per lane
+__device__ __forceinline__ double lane_bcast(double v, int src)
+{
+   const long long x = __double_as_longlong(v);
+   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, src);
+   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), src);
+   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// nvar x nparam -> nparam x nvar (32 x 32 tiles through LDS)
+__global__ void __launch_bounds__(256) k_transpose_deps(const double2 *__restrict__ in,
+                                                        double2 *__restrict__ out, int nvar,
+                                                        int nparam)
+{
+   __shared__ double2 tile[32][33];
+   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+   const int p0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+   for (int j = ty; j < 32; j += 8)
+   {
+      const int v = v0 + j, pi = p0 + tx;
+      if (v < nvar && pi < nparam) { tile[j][tx] = in[(size_t)v * nparam + pi]; }
+   }
+   __syncthreads();
+   for (int j = ty; j < 32; j += 8)
+   {
+      const int pi = p0 + j, v = v0 + tx;
+      if (v < nvar && pi < nparam) { out[(size_t)pi * nvar + v] = tile[tx][j]; }
+   }
+}
 
 __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
 {
@@ -109,29 +140,43 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
          }
       }
    }
-   for (int v = 0; v < V.nvar; ++v)
+   // variants in blocks of 64: lane j holds the eps offsets of variant
+   // v0 + j for every contributor (one coalesced load each from the
+   // parameter-major copy), broadcast by readlane inside the variant loop
+   for (int v0 = 0; v0 < V.nvar; v0 += 64)
    {
-      wave_sync_lds();
-      for (int s = lane; s < len; s += 64) { buf[s] = row0[s]; }
-      wave_sync_lds();
-      const double2 *dv = V.deps + (size_t)v * V.nparam;
+      double2 dk[VMAXC];
 #pragma unroll
       for (int k = 0; k < VMAXC; ++k)
       {
-         if (k < nc)
-         {
-            const double2 t = cmul(dv[pidx[k]], m[k]);
-            if (use[k])
-            {
-               double2 x = buf[sl[k]];
-               x.x += t.x;
-               x.y += t.y;
-               buf[sl[k]] = x;
-            }
-            wave_sync_lds();
-         }
+         dk[k] = (k < nc && v0 + lane < V.nvar)
+                    ? V.depsT[(size_t)pidx[k] * V.nvar + v0 + lane] : make_double2(0.0, 0.0);
       }
-      for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = buf[s]; }
+      const int v1 = min(v0 + 64, V.nvar);
+      for (int v = v0; v < v1; ++v)
+      {
+         wave_sync_lds();
+         for (int s = lane; s < len; s += 64) { buf[s] = row0[s]; }
+         wave_sync_lds();
+#pragma unroll
+         for (int k = 0; k < VMAXC; ++k)
+         {
+            if (k < nc)
+            {
+               const double2 d = make_double2(lane_bcast(dk[k].x, v - v0), lane_bcast(dk[k].y, v - v0));
+               const double2 t = cmul(d, m[k]);
+               if (use[k])
+               {
+                  double2 x = buf[sl[k]];
+                  x.x += t.x;
+                  x.y += t.y;
+                  buf[sl[k]] = x;
+               }
+               wave_sync_lds();
+            }
+         }
+         for (int s = lane; s < len; s += 64) { V.out[(size_t)v * V.nnz + base + s] = buf[s]; }
+      }
    }
 }
 
