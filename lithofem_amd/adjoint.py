@@ -26,6 +26,8 @@ mean squared deviation of |E|^2 on a slice from a target.
 
 from __future__ import annotations
 
+import time
+
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
@@ -152,31 +154,44 @@ class EpsForwardModel:
         self.ess_mask[ad.ess] = True
         self.lu = None
         self.u = None
+        self.times = {}          # seconds of the last solve / adjoint, by phase
 
     def close(self):
         self.g.close()
 
     def solve(self, eps_params: np.ndarray) -> np.ndarray:
+        t = [time.perf_counter()]
         ee = self.eps0.copy()
         ee[self.params] = eps_params
         self.g.set_eps_elem(ee)
         self.g.compute_local()
         self.g.scatter_eliminate(True)
-        vre, vim = self.g.download_values()
+        vre, vim = self.g.download_values()       # waits for the assembly kernels
+        t.append(time.perf_counter())
         if self.rhs_col >= 0:
             b = self.g.assemble_rhs(eliminate=True)[self.rhs_col]
         else:
             b = np.zeros(self.ad.height, dtype=np.complex128)
+        t.append(time.perf_counter())
         self.lu = HostLU(self.I, self.J, vre + 1j * vim, self.ad.height)
+        t.append(time.perf_counter())
         self.u = self.lu.solve(b)
+        t.append(time.perf_counter())
+        self.times = dict(zip(("assembly_and_download", "rhs", "factorisation", "solve"),
+                              map(float, np.diff(t))))
         return self.u
 
     def adjoint_sensitivity(self, grad_u_conj: np.ndarray) -> np.ndarray:
         """s_e for every parameter element, given c = conj(dL/du*) (so that
         A^T lam = c)."""
+        t0 = time.perf_counter()
         lam = self.lu.solve_transpose(grad_u_conj)
         lam[self.ess_mask] = 0.0
-        return self.g.eps_sensitivity(self.params, lam, self.u, self.rhs_col)
+        t1 = time.perf_counter()
+        s = self.g.eps_sensitivity(self.params, lam, self.u, self.rhs_col)
+        t2 = time.perf_counter()
+        self.times.update(transpose_solve=t1 - t0, sensitivity=t2 - t1)
+        return s
 
 
 def make_torch_function():

@@ -303,12 +303,16 @@ def adjoint(log, n_xy=8, nz_scale=0.67, n_fd=2) -> dict:
     t0 = time.perf_counter()
     u = F.apply(eps, model)
     t_fwd = time.perf_counter() - t0
-    log(f"adjoint: forward assembly + factorisation + solve {t_fwd:.2f} s")
+    fwd_phases = dict(model.times)
+    log(f"adjoint: forward assembly + factorisation + solve {t_fwd:.2f} s "
+        + " ".join(f"{k} {v:.3f}" for k, v in fwd_phases.items()))
     Jv = obj(u)
     t0 = time.perf_counter()
     Jv.backward()
     t_bwd = time.perf_counter() - t0
-    log(f"adjoint: backward (transpose solve + sensitivity kernel) {t_bwd:.3f} s")
+    bwd_phases = {k: model.times[k] for k in ("transpose_solve", "sensitivity")}
+    log(f"adjoint: backward (transpose solve + sensitivity kernel) {t_bwd:.3f} s "
+        + " ".join(f"{k} {v:.3f}" for k, v in bwd_phases.items()))
     grad = eps.grad.numpy()
     rels = []
     picks = np.random.default_rng(4).choice(len(params), n_fd, replace=False)
@@ -330,6 +334,8 @@ def adjoint(log, n_xy=8, nz_scale=0.67, n_fd=2) -> dict:
             "value": (t_fwd + t_bwd) / t_fwd, "unit": "x forward", "higher_is_better": False,
             "detail": {"elements": ad.ne, "dof": ad.height, "params": len(params),
                        "forward_s": t_fwd, "backward_s": t_bwd,
+                       "forward_phases_s": fwd_phases, "backward_phases_s": bwd_phases,
+                       "host_lu": "scipy SuperLU (splu, MMD_AT_PLUS_A) on the host",
                        "sensitivity_kernel_ms": adj_ms, "objective": float(Jv.detach()),
                        "fd_rel_errors": rels}}
 
