@@ -15,7 +15,9 @@ Prints one JSON line.  `value` is elements/s of the whole step; `roofline`
 is the kernel with the largest share of the step against its own roofline
 (HBM bytes or canonical FP64 FLOPs), every kernel's figures are under
 `detail.kernels`; `cpu_baseline` is the C oracle (the restated reference
-reassembler) timed on one host core over a bounded sample.
+reassembler) timed on one host core over a bounded sample, and
+`detail.cpu_baseline_threads` the same oracle over all usable host threads
+(at most 16).
 """
 
 from __future__ import annotations
@@ -77,7 +79,25 @@ def cpu_baseline(wl, I, J, seconds: float = 12.0) -> dict:
     return dict(value=n / t_local, unit="elements/s", cores=1, kind="port",
                 sample=f"elements [{e0}, {e0 + n}) of the same mesh ({n} of {ad.ne}), "
                        f"oracle local matrices + face transform, {t_local:.1f} s",
-                _Ae=Ae, _e0=e0)
+                _Ae=Ae, _e0=e0, _orc=orc, _per=t_local / n)
+
+
+def cpu_baseline_threads(cb: dict, ne: int, seconds: float = 6.0) -> dict:
+    """The same oracle over all host threads the process may use (at most 16):
+    one contiguous element block per thread (each thread writes its own
+    output; the C calls release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    nt = max(1, min(16, len(os.sched_getaffinity(0))))
+    orc = cb["_orc"]
+    n = int(min(ne, max(nt * 64, nt * seconds / cb["_per"])))
+    e0 = (ne - n) // 2
+    cuts = [e0 + (n * i) // nt for i in range(nt + 1)]
+    with ThreadPoolExecutor(nt) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda i: orc.elements(cuts[i], cuts[i + 1]), range(nt)))
+        t = time.perf_counter() - t0
+    return dict(value=n / t, unit="elements/s", cores=nt, kind="port",
+                sample=f"elements [{e0}, {e0 + n}) in {nt} contiguous blocks, {t:.1f} s")
 
 
 def main() -> int:
@@ -294,6 +314,9 @@ def main() -> int:
         out["cpu_baseline"] = {k: v for k, v in cb.items() if not k.startswith("_")}
         out["cpu_baseline"]["sample_parity_maxnorm"] = err
         out["cpu_baseline"]["host_cpu"] = _cpu_model()
+        ct = cpu_baseline_threads(cb, ad.ne)
+        log(f"cpu baseline, {ct['cores']} threads: {ct['value']:.1f} elements/s")
+        out["detail"]["cpu_baseline_threads"] = ct
     g.close()
     print(json.dumps(out), flush=True)
     return 0
