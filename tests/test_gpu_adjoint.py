@@ -76,3 +76,31 @@ def test_adjoint_gradient_matches_central_differences():
                 assert rel < 1e-5, (i, dirn, an, fd)
     finally:
         model.close()
+
+
+def test_region_sum_matches_two_solve_difference():
+    """The reference has only region eps, so the quantity it could check is
+    the region sum: sum of g_e over one region against the central
+    difference of J when that whole region's eps moves (real and imaginary
+    directions)."""
+    model, obj, eps0, params, ad = _setup()
+    try:
+        F = make_torch_function()
+        eps = torch.tensor(eps0, dtype=torch.complex128, requires_grad=True)
+        J = obj(F.apply(eps, model))
+        J.backward()
+        grad = eps.grad.numpy()
+        for attr in (2, 3):
+            sel = ad.attr[params] == attr
+            assert sel.any()
+            for dirn in (1.0, 1j):
+                h = 1e-6 * abs(eps0[sel]).max()
+                d = np.where(sel, h * dirn, 0.0)
+                fd = (_J(model, obj, eps0 + d) - _J(model, obj, eps0 - d)) / (2 * h)
+                an = (np.conj(grad[sel]) * dirn).real.sum()
+                rel = abs(fd - an) / max(abs(an), 1e-30)
+                print(f"  region {attr} dir {dirn}: sum g_e {an:.9e} two-solve {fd:.9e} rel {rel:.2e}",
+                      flush=True)
+                assert rel < 1e-5, (attr, dirn, an, fd)
+    finally:
+        model.close()
