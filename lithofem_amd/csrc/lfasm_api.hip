@@ -593,9 +593,12 @@ hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
    if (T.lda % 8 == 0 && T.lda > T.nd && !T.out_compact)
    {
       // padded rows: the column-per-lane kernel with whole-line stores
+      // element coefficients first, then the entries
+      LF_TRY(launch_tensor_coef(h, T));
       const dim3 grid((unsigned)((h->nconst + TEN21_EPC - 1) / TEN21_EPC),
                       (unsigned)((T.npair * T.lda + 255) / 256));
-      hipLaunchKernelGGL(k_local_tensor21, grid, dim3(256), 0, h->stream, T);
+      hipLaunchKernelGGL(k_local_tensor21, grid, dim3(256), 0, h->stream, T, T.elist,
+                         (const double *)h->coef, T.Ae);
       return hipGetLastError();
    }
    const int epar = T.nb > 128 ? 1 : 256 / T.nb;
@@ -1096,7 +1099,7 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
       for (int e : cl) { ec[(size_t)e] = 1; }
       LF_HIP(upload(h, h->eclass, ec.data(), ec.size()));
       h->eclass_host = ec;
-      if (m0.p <= 3) { LF_HIP(hipMalloc((void **)&h->coef, ne * TEN_NC * sizeof(double))); }
+      LF_HIP(hipMalloc((void **)&h->coef, ne * TEN_NC * sizeof(double)));
       const int npair = (m0.nd + 1) / 2;
       const size_t nR = (size_t)npair * npair * NTENS * 4;
       LF_HIP(hipMalloc((void **)&h->Rten, nR * sizeof(double)));
@@ -1215,7 +1218,7 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
       release_global(h);
       return h->fail(LFASM_E_PATTERN, "an element (row, col) pair is missing from the CSR pattern");
    }
-   if (h->coef)
+   if (h->coef && h->p <= 3)
    {
       const int rc = build_fused_units(h, I, dof, rc_ptr, rc_list, sign, ess);
       if (rc != LFASM_OK) { release_global(h); return rc; }

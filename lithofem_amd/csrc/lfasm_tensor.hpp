@@ -286,36 +286,40 @@ __global__ void __launch_bounds__(256) k_local_tensor(TensorParams P, int epar)
 // The column face pair (l, l^1) sits in lanes (t, t^1).  Every entry takes
 // exactly the FMAs of k_local_tensor, so both give bit-identical values.
 typedef double ten_dv2 __attribute__((ext_vector_type(2)));
-constexpr int TEN21_EPC = 64;   // elements per workgroup
+constexpr int TEN21_EPC = 16;   // elements per workgroup
 
-__global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P)
+// The element coefficients come precomputed from k_tensor_coef (28 doubles
+// per element, the face orientations packed into the last) and are read
+// with wave-uniform loads: they sit in scalar registers and feed the FMAs
+// directly, so the vector registers hold only the 42 tensor values and the
+// entries (4 waves per SIMD) and no LDS staging or barrier sits between
+// element batches.  Measured at p = 3 on 58,752 elements: 0.43 ms against
+// 0.48 ms for coefficients staged in LDS by each workgroup (64 elements per
+// workgroup; fewer made the staging version slower).
+__global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P,
+                                                        const int *__restrict__ elist,
+                                                        const double *__restrict__ coef,
+                                                        double2 *__restrict__ Ae)
 {
-   __shared__ double coef[TEN21_EPC * TEN_NC];
-   __shared__ int sfo[TEN21_EPC * 4];
-   __shared__ int sel[TEN21_EPC];
    const int tid = threadIdx.x;
    const int c0 = blockIdx.x * TEN21_EPC;
    const int cnt = min(TEN21_EPC, P.nlist - c0);
-   if (tid < cnt)
-   {
-      const int e = P.elist[c0 + tid];
-      sel[tid] = P.out_compact ? c0 + tid : e;
-      tensor_coef(P, e, coef + tid * TEN_NC);
-      if (P.p >= 2)
-      {
-#pragma unroll
-         for (int f = 0; f < 4; ++f) { sfo[tid * 4 + f] = P.fo[(size_t)e * 4 + f]; }
-      }
-   }
-   __syncthreads();
    const int nd = P.nd, lda = P.lda;
+   // face-transform table in LDS (its entries are picked per lane)
+   __shared__ double TT[24];
+   if (tid < 24) { TT[tid] = P.TInv[tid]; }
+   __syncthreads();
+   // waves with no entry of the element matrices (past the last row pair)
+   if (__builtin_amdgcn_readfirstlane((int)(blockIdx.y * 256 + (tid & ~63))) >= P.npair * lda)
+   {
+      return;
+   }
    const int blk = blockIdx.y * 256 + tid;
    const bool live = blk < P.npair * lda;
    const int bb = live ? blk : 0;
    const int rp = bb / lda, l = bb - rp * lda;
    const int k0 = 2 * rp, odd = l & 1;
    const bool col_ok = l < nd;
-   // blocked reference tensors: 2x2 block (rp, l/2), entries (k0 + i, l)
    const double *Rb = P.R + ((size_t)rp * P.npair + (col_ok ? l >> 1 : 0)) * NTENS * 4;
    double r[NTENS][2];
 #pragma unroll
@@ -330,7 +334,8 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P)
 
    for (int i = 0; i < cnt; ++i)
    {
-      const double *c = coef + i * TEN_NC;
+      const int e = elist[c0 + i];
+      const double *c = coef + (size_t)e * TEN_NC;
       double re[2], im[2];
 #pragma unroll
       for (int s = 0; s < 2; ++s)
@@ -348,10 +353,8 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P)
          re[s] = x;
          im[s] = y;
       }
-      // dual face transform: columns (partner column in lane ^ 1; every
-      // lane takes part in the exchange), then rows (in the thread)
-      const int foc = fc >= 0 ? sfo[i * 4 + fc] : 0;
-      const double *Tc = P.TInv + 4 * foc;
+      const int packed = (int)c[27];
+      const double *Tc = TT + 4 * (fc >= 0 ? (packed >> (3 * fc)) & 7 : 0);
 #pragma unroll
       for (int s = 0; s < 2; ++s)
       {
@@ -372,14 +375,14 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P)
       }
       if (fr >= 0)
       {
-         const double *Ti = P.TInv + 4 * sfo[i * 4 + fr];
+         const double *Ti = TT + 4 * ((packed >> (3 * fr)) & 7);
          const double ar = re[0], ai = im[0], br = re[1], bi = im[1];
          re[0] = pair_lo(Ti, ar, br);
          im[0] = pair_lo(Ti, ai, bi);
          re[1] = pair_hi(Ti, ar, br);
          im[1] = pair_hi(Ti, ai, bi);
       }
-      ten_dv2 *A = reinterpret_cast<ten_dv2 *>(P.Ae) + (size_t)sel[i] * nd * lda;
+      ten_dv2 *A = reinterpret_cast<ten_dv2 *>(Ae) + (size_t)(P.out_compact ? c0 + i : e) * nd * lda;
       const ten_dv2 v0 = col_ok ? ten_dv2{re[0], im[0]} : ten_dv2{0.0, 0.0};
       const ten_dv2 v1 = col_ok ? ten_dv2{re[1], im[1]} : ten_dv2{0.0, 0.0};
       if (st0) { __builtin_nontemporal_store(v0, A + (size_t)k0 * lda + l); }
