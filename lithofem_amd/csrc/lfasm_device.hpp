@@ -882,6 +882,61 @@ __device__ __forceinline__ void wave_sync_lds()
 // contributor (ascending element order) so the sums are deterministic.
 constexpr int SCATTER_CG = 4;
 
+// M contributors (list entries cg .. cg+M-1) of one CSR row into the row
+// buffer: every load of the group is issued before the first use, with no
+// branch between them, so they are in flight together; the accumulation
+// then runs in ascending element order (deterministic sums).
+template <int M>
+__device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *buf, int cg, int lane,
+                                              unsigned ess_bit)
+{
+   const int nd = S.nd;
+   int ea[M];   // e*nd + a
+#pragma unroll
+   for (int u = 0; u < M; ++u) { ea[u] = S.rc_list[cg + u]; }
+   for (int b0 = 0; b0 < nd; b0 += 64)
+   {
+      const int b = b0 + lane;
+      const int bc = min(b, nd - 1);   // lanes past the row load a valid entry, unused
+      dbl2v x[M];
+      int sl[M];
+      unsigned fa[M], fb[M];
+#pragma unroll
+      for (int u = 0; u < M; ++u)
+      {
+         const int e = ea[u] / nd;
+         if (S.emask)
+         {
+            // two uniform 8-byte loads instead of a byte per lane
+            const uint64_t sm = S.emask[2 * (size_t)e], em = S.emask[2 * (size_t)e + 1];
+            fa[u] = (unsigned)((sm >> (ea[u] - e * nd)) & 1);
+            fb[u] = (unsigned)(((sm >> bc) & 1) | (((em >> bc) & 1) << 1));
+         }
+         else
+         {
+            fa[u] = S.eflag[ea[u]];
+            fb[u] = S.eflag[(size_t)e * nd + bc];
+         }
+         sl[u] = S.slot[(size_t)ea[u] * S.sld + bc];
+         x[u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2v *>(S.Ae) +
+                                           (size_t)ea[u] * S.lda + bc);
+      }
+#pragma unroll
+      for (int u = 0; u < M; ++u)
+      {
+         if (b < nd && (fb[u] & ess_bit) == 0)
+         {
+            const bool neg = ((fa[u] ^ fb[u]) & 1) != 0;
+            double2 t = buf[sl[u]];
+            t.x += neg ? -x[u].x : x[u].x;
+            t.y += neg ? -x[u].y : x[u].y;
+            buf[sl[u]] = t;
+         }
+         wave_sync_lds();
+      }
+   }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
 {
    extern __shared__ double2 rowbuf[];
@@ -895,65 +950,17 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
    for (int s = lane; s < len; s += 64) { buf[s] = make_double2(0.0, 0.0); }
    wave_sync_lds();
    const bool row_ess = S.eliminate && S.is_ess[r];
-   const int nd = S.nd;
-   const unsigned char ess_bit = S.eliminate ? 2 : 0;
+   const unsigned ess_bit = S.eliminate ? 2u : 0u;
    if (!row_ess)
    {
       const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
       for (int cg = c0; cg < c1; cg += SCATTER_CG)
       {
-         for (int b0 = 0; b0 < nd; b0 += 64)
-         {
-            const int b = b0 + lane;
-            double2 v[SCATTER_CG];
-            int sl[SCATTER_CG];
-            bool use[SCATTER_CG];
-#pragma unroll
-            for (int u = 0; u < SCATTER_CG; ++u)
-            {
-               const int c = cg + u;
-               use[u] = false;
-               v[u] = make_double2(0.0, 0.0);
-               sl[u] = 0;
-               if (c < c1 && b < nd)
-               {
-                  const int ea = S.rc_list[c];          // e*nd + a
-                  const int e = ea / nd;
-                  unsigned char fa, fb;
-                  if (S.emask)
-                  {
-                     // two uniform 8-byte loads instead of a byte per lane
-                     const uint64_t sm = S.emask[2 * (size_t)e], em = S.emask[2 * (size_t)e + 1];
-                     const int a = ea - e * nd;
-                     fa = (unsigned char)((sm >> a) & 1);
-                     fb = (unsigned char)(((sm >> b) & 1) | (((em >> b) & 1) << 1));
-                  }
-                  else
-                  {
-                     fa = S.eflag[ea];
-                     fb = S.eflag[(size_t)e * nd + b];
-                  }
-                  use[u] = (fb & ess_bit) == 0;
-                  sl[u] = S.slot[(size_t)ea * S.sld + b];
-                  const dbl2v x = __builtin_nontemporal_load(
-                     reinterpret_cast<const dbl2v *>(S.Ae) + (size_t)ea * S.lda + b);
-                  const bool neg = ((fa ^ fb) & 1) != 0;
-                  v[u] = neg ? make_double2(-x.x, -x.y) : make_double2(x.x, x.y);
-               }
-            }
-#pragma unroll
-            for (int u = 0; u < SCATTER_CG; ++u)
-            {
-               if (use[u])
-               {
-                  double2 t = buf[sl[u]];
-                  t.x += v[u].x;
-                  t.y += v[u].y;
-                  buf[sl[u]] = t;
-               }
-               wave_sync_lds();
-            }
-         }
+         const int m = min(SCATTER_CG, c1 - cg);
+         if (m == 4) { scatter_group<4>(S, buf, cg, lane, ess_bit); }
+         else if (m == 3) { scatter_group<3>(S, buf, cg, lane, ess_bit); }
+         else if (m == 2) { scatter_group<2>(S, buf, cg, lane, ess_bit); }
+         else { scatter_group<1>(S, buf, cg, lane, ess_bit); }
       }
    }
    else
@@ -966,8 +973,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
    }
    // each store instruction writes 64 consecutive entries (whole lines):
    // non-temporal stores measured faster here (1.057 vs 1.138 ms at 1.35M
-   // DOF); the element-matrix kernels, whose lanes write every other entry
-   // per instruction, keep ordinary stores
+   // DOF)
    for (int s = lane; s < len; s += 64)
    {
       const double2 t = buf[s];
