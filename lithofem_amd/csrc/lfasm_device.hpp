@@ -880,7 +880,7 @@ __device__ __forceinline__ void wave_sync_lds()
 // Contributors are processed in groups of CG: the loads of a group are
 // issued together, the LDS accumulation then runs contributor by
 // contributor (ascending element order) so the sums are deterministic.
-constexpr int SCATTER_CG = 4;
+constexpr int SCATTER_CG = 4;   // 6 and 8 measured slower at p = 3
 
 // M contributors (list entries cg .. cg+M-1) of one CSR row into the row
 // buffer: every load of the group is issued before the first use, with no
@@ -904,6 +904,13 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
 #pragma unroll
       for (int u = 0; u < M; ++u)
       {
+         sl[u] = S.slot[(size_t)ea[u] * S.sld + bc];
+         x[u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2v *>(S.Ae) +
+                                           (size_t)ea[u] * S.lda + bc);
+      }
+#pragma unroll
+      for (int u = 0; u < M; ++u)
+      {
          const int e = ea[u] / nd;
          if (S.emask)
          {
@@ -917,9 +924,6 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
             fa[u] = S.eflag[ea[u]];
             fb[u] = S.eflag[(size_t)e * nd + bc];
          }
-         sl[u] = S.slot[(size_t)ea[u] * S.sld + bc];
-         x[u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2v *>(S.Ae) +
-                                           (size_t)ea[u] * S.lda + bc);
       }
 #pragma unroll
       for (int u = 0; u < M; ++u)
@@ -934,6 +938,19 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
          }
          wave_sync_lds();
       }
+   }
+}
+
+// scatter_group<m> for a group of m <= M contributors (m is wave-uniform)
+template <int M>
+__device__ __forceinline__ void scatter_dispatch(const ScatterParams &S, double2 *buf, int cg, int m,
+                                                 int lane, unsigned ess_bit)
+{
+   if constexpr (M == 1) { scatter_group<1>(S, buf, cg, lane, ess_bit); }
+   else
+   {
+      if (m == M) { scatter_group<M>(S, buf, cg, lane, ess_bit); }
+      else { scatter_dispatch<M - 1>(S, buf, cg, m, lane, ess_bit); }
    }
 }
 
@@ -956,11 +973,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
       const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
       for (int cg = c0; cg < c1; cg += SCATTER_CG)
       {
-         const int m = min(SCATTER_CG, c1 - cg);
-         if (m == 4) { scatter_group<4>(S, buf, cg, lane, ess_bit); }
-         else if (m == 3) { scatter_group<3>(S, buf, cg, lane, ess_bit); }
-         else if (m == 2) { scatter_group<2>(S, buf, cg, lane, ess_bit); }
-         else { scatter_group<1>(S, buf, cg, lane, ess_bit); }
+         scatter_dispatch<SCATTER_CG>(S, buf, cg, min(SCATTER_CG, c1 - cg), lane, ess_bit);
       }
    }
    else
