@@ -1173,6 +1173,15 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
       release_global(h);
       return h->fail(LFASM_E_UNSUPPORTED, "row length %d > 65535", maxrow);
    }
+   // the row scatter keeps 4 rows per workgroup in LDS
+   int lds_max = 0;
+   LF_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+   if ((size_t)4 * maxrow * sizeof(double2) > (size_t)lds_max)
+   {
+      release_global(h);
+      return h->fail(LFASM_E_UNSUPPORTED, "row length %d > %d (LDS row buffers of the scatter)",
+                     maxrow, lds_max / (int)(4 * sizeof(double2)));
+   }
    h->maxrow = maxrow;
    std::vector<unsigned char> ess((size_t)n, 0);
    if (is_ess) { std::memcpy(ess.data(), is_ess, (size_t)n); }
@@ -1674,6 +1683,16 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    if (!h || !h->nparam) { return h ? h->fail(LFASM_E_STATE, "variants_prepare first") : LFASM_E_STATE; }
    if (nvar <= 0 || !deps) { return h->fail(LFASM_E_ARG, "no variants"); }
    LF_HIP(hipSetDevice(h->device));
+   {
+      // the variant row kernels keep 2 rows per wavefront, 4 wavefronts, in LDS
+      int lds_max = 0;
+      LF_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+      if ((size_t)8 * h->maxrow * sizeof(double2) > (size_t)lds_max)
+      {
+         return h->fail(LFASM_E_UNSUPPORTED, "row length %d > %d (LDS row buffers of the variant kernel)",
+                        h->maxrow, lds_max / (int)(8 * sizeof(double2)));
+      }
+   }
    if (nvar != h->nvar)
    {
       dfree(h->var_vals);
