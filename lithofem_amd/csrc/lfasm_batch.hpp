@@ -36,6 +36,7 @@ struct VariantParams {
    const double2 *base;          // nnz
    double2 *out;                 // nvar x nnz
    int eliminate, maxrow;
+   const unsigned char *vcopied; // per row: 1 = written by k_variant_copy (or NULL)
 };
 
 // Register-resident variant of k_variant_rows for nd <= 64 and at most
@@ -75,6 +76,29 @@ __global__ void __launch_bounds__(256) k_transpose_deps(const double2 *__restric
    }
 }
 
+// Long runs of consecutive rows without a parameter contributor are the
+// base rows in every variant and contiguous ranges of the CSR values:
+// copied here in chunks (one workgroup per chunk, whole-line non-temporal
+// stores except at the chunk ends).  The row kernels skip those rows
+// (vcopied[r] = 1) and keep the short runs, whose stores share their first
+// and last lines with neighbouring rows written at about the same time.
+__global__ void __launch_bounds__(256) k_variant_copy(const int2 *__restrict__ chunks,
+                                                      const double2 *__restrict__ base,
+                                                      double2 *__restrict__ out, int64_t nnz,
+                                                      int nvar)
+{
+   const int2 c = chunks[blockIdx.x];
+   for (int i = c.x + (int)threadIdx.x; i < c.y; i += 256)
+   {
+      const double2 x = base[i];
+      for (int v = 0; v < nvar; ++v)
+      {
+         __builtin_nontemporal_store(dbl2v{x.x, x.y},
+                                     reinterpret_cast<dbl2v *>(out + (size_t)v * nnz + i));
+      }
+   }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
 {
    extern __shared__ double2 vbuf[];
@@ -84,6 +108,7 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
    if (r >= V.n) { return; }
    double2 *row0 = vbuf + (size_t)wave * 2 * V.maxrow;
    double2 *buf = row0 + V.maxrow;
+   if (V.vcopied && V.vcopied[r]) { return; }   // k_variant_copy
    const int base = V.Ip[r];
    const int len = V.Ip[r + 1] - base;
    const int c0 = V.prc_ptr[r], c1 = V.prc_ptr[r + 1];
@@ -188,6 +213,7 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
    if (r >= V.n) { return; }
    double2 *row0 = vbuf + (size_t)wave * 2 * V.maxrow;
    double2 *buf = row0 + V.maxrow;
+   if (V.vcopied && V.vcopied[r]) { return; }   // k_variant_copy
    const int base = V.Ip[r];
    const int len = V.Ip[r + 1] - base;
    const int c0 = V.prc_ptr[r], c1 = V.prc_ptr[r + 1];
