@@ -82,6 +82,9 @@ def test_multi_mesh_batch_equals_single(p):
 
 @pytest.mark.parametrize("p", [2, 3])
 def test_eps_variants_equal_plain_assembly(p):
+    # at this size both variant kernels run: the row kernel and the range
+    # copy of long runs of parameter-free rows (2 runs of >= 32,768 values
+    # at p = 2, 6 at p = 3)
     c = line_grating(nx=4, ny=3, nz_scale=0.15, theta=6.0)
     pb = c.problem
     ad = build_asm_data(c.mesh, p)
