@@ -175,6 +175,9 @@ def main() -> int:
         tuned = tune_local(g)
         log(f"autotune: best kernel {tuned['best']}: " +
             ", ".join(f"{k}={v:.3f}ms" for k, v in tuned["ms"].items()))
+    else:
+        # the captured step launches the handle's default kernel
+        g.set_local_kernel(args.kernel)
 
     def step():
         g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=args.kernel)

@@ -9,6 +9,7 @@
 #include "lfasm_batch.hpp"
 #include "lfasm_tensor.hpp"
 #include "lfasm_pattern.hpp"
+#include "lfasm_sym.hpp"
 #include "../../include/lithofem_asm.h"
 
 #include <hip/hip_runtime.h>
@@ -343,9 +344,29 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
    return hipGetLastError();
 }
 
+// Symmetric-split kernel (nd <= 48 with 3 tiles; CROSS0 and CROSS1 both on
+// or both off).
+template <int NBUF, int WPEU>
+hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
+{
+   const int nqt = P.rc.nq + P.rm.nq;
+   const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + NBUF * SYM_STG) * sizeof(double);
+   hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_sym<NBUF, WPEU>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+   if (e != hipSuccess) { return e; }
+   hipLaunchKernelGGL((k_local_mfma_sym<NBUF, WPEU>), dim3((unsigned)P.nout), dim3(SYM_THREADS),
+                      shmem, h->stream, P, nqt);
+   return hipGetLastError();
+}
+
+bool sym_applies(const LocalParams &P)
+{
+   return P.nd > 32 && P.nd <= 48 && ((P.mflags >> 1) & 1) == ((P.mflags >> 2) & 1);
+}
+
 // MFMA launch variants per tile count (element-per-block count, waves per
-// EU, pipelined or not).  Variant 0 of each list is the default; the
-// launch autotuner (lithofem_amd.tune) times all of them.
+// EU, pipelined or not, symmetric split).  Variant 0 of each list is the
+// default; the launch autotuner (lithofem_amd.tune) times all of them.
 constexpr int N_MFMA_VARIANTS = LFASM_N_MFMA_VARIANTS;
 
 hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, int v)
@@ -356,18 +377,26 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
    case 1 * 16 + 1: return launch_mfma_pipe<1, 2, 2>(h, P);
    case 1 * 16 + 2: return launch_mfma<1, 1, 4, 2>(h, P);
    case 1 * 16 + 3: return launch_mfma_pipe<1, 4, 4>(h, P);
+   case 1 * 16 + 4: return launch_mfma_pipe<1, 4, 2>(h, P);
+   case 1 * 16 + 5: return launch_mfma_pipe<1, 2, 2>(h, P);
    case 2 * 16 + 0: return launch_mfma_pipe<2, 2, 2>(h, P);
    case 2 * 16 + 1: return launch_mfma_pipe<2, 1, 2>(h, P);
    case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P);
    case 2 * 16 + 3: return launch_mfma<2, 2, 4, 2>(h, P);
-   case 3 * 16 + 0: return launch_mfma_pipe<3, 1, 2>(h, P);
+   case 2 * 16 + 4: return launch_mfma_pipe<2, 2, 2>(h, P);
+   case 2 * 16 + 5: return launch_mfma_pipe<2, 1, 2>(h, P);
+   case 3 * 16 + 0: return sym_applies(P) ? launch_mfma_sym<1, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 1: return launch_mfma_pipe<3, 2, 2>(h, P);
    case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
+   case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<2, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
+   case 3 * 16 + 5: return launch_mfma_pipe<3, 1, 2>(h, P);
    case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
    case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
    case 6 * 16 + 3: return launch_mfma_pipe<6, 1, 2>(h, P);
+   case 6 * 16 + 4: return launch_mfma_pipe<6, 1, 2>(h, P);
+   case 6 * 16 + 5: return launch_mfma_pipe<6, 1, 1>(h, P);
    default: return hipErrorInvalidValue;
    }
 }

@@ -1,0 +1,342 @@
+// Quadrature kernel for nd <= 48 that uses the symmetry of the element
+// matrix (solver/asm_gpu.inc:70-214 computes every entry).
+//
+// Ae = sum_q [v c]^T D_q [v c] over the mass rule plus the curl-curl sum.
+// The curl-curl, mass and -K Lam^-1 K coefficients are symmetric; the
+// MixedVectorCurl / MixedVectorWeakCurl pair couples v and c with
+// antisymmetric coefficients (CROSS1 = -CROSS0^T, the C3..C5 rows of
+// mass_coef are minus the transpose of the c-terms of C0..C2).  So
+//     Ae = S + P - P^T,
+// S = sum_cc c^T (w Lam^-1) c + sum_m v^T Dvv v   (symmetric),
+// P = sum_m v^T X c                                (the CROSS0 coupling),
+// and only the tiles of S on and above the diagonal are formed.  With three
+// 16-wide tiles (nd <= 48) a wavefront owns the tile pair {(ti, tj), (tj, ti)}
+// and one diagonal tile td:
+//     Ae(ti, tj) = S(ti, tj) + P(ti, tj) - P(tj, ti)^T
+//     Ae(tj, ti) = S(ti, tj)^T + P(tj, ti) - P(ti, tj)^T
+//     Ae(td, td) = S(td, td) + P(td, td) - P(td, td)^T
+// 216 MFMAs per wavefront instead of 270 at p = 3 with k_par (cc rule 11
+// points, mass rule 24).  The K stream stages per mass point only the value
+// rows v (the curl rows enter through X c), and per row the symmetric
+// coefficient row CS and the cross row CP.  Masks holding only one of
+// CROSS0 / CROSS1 are not antisymmetric and go to the general kernels.
+#pragma once
+
+#include "lfasm_device.hpp"
+
+namespace lfasm {
+
+constexpr int SYM_NP = 48;                        // 3 tiles of 16 columns
+constexpr int SYM_ROWS = 12;                      // 4 points x 3 rows per chunk
+constexpr int SYM_STG = 5 * SYM_ROWS * SYM_NP;    // A, CSr, CSi, CPr, CPi
+constexpr int SYM_THREADS = 192;                  // 3 wavefronts per element
+
+struct SymShape {
+   int nq_cc, nq_m, nch_cc, nch_m;
+   int cross;   // CROSS0 and CROSS1 both on
+};
+
+__device__ __forceinline__ SymShape sym_shape(const LocalParams &P)
+{
+   SymShape s;
+   s.nq_cc = P.use_cc ? P.rc.nq : 0;
+   s.nq_m = (P.mflags & 15) ? P.rm.nq : 0;
+   s.nch_cc = (s.nq_cc + 3) / 4;
+   s.nch_m = (s.nq_m + 3) / 4;
+   s.cross = (P.mflags & 6) == 6;
+   return s;
+}
+
+// One (point, basis k) item of a chunk: thread t forms point t / 48, basis
+// t % 48 (192 threads, 4 points x 48 columns).
+struct SymItem {
+   double h[6];
+   int q, k, row0;
+   bool valid, live, mass;
+};
+
+__device__ __forceinline__ SymItem sym_load_item(const LocalParams &P, const SymShape &ss, int ch,
+                                                 int tid)
+{
+   SymItem it;
+#pragma unroll
+   for (int i = 0; i < 6; ++i) { it.h[i] = 0.0; }
+   it.mass = ch >= ss.nch_cc;
+   const int c = it.mass ? ch - ss.nch_cc : ch;
+   const int nq = it.mass ? ss.nq_m : ss.nq_cc;
+   const int qi = tid / SYM_NP;
+   it.k = tid - qi * SYM_NP;
+   const int q = c * 4 + qi;
+   it.valid = q < nq;
+   it.q = it.valid ? q : 0;
+   it.row0 = qi * 3;
+   it.live = it.valid && it.k < P.nd;
+   if (it.live)
+   {
+      const RuleDev &r = it.mass ? P.rm : P.rc;
+      const double *t0 = (it.mass ? r.vs : r.cs) + ((size_t)it.q * P.nd + it.k) * 3;
+      it.h[0] = t0[0];
+      it.h[1] = t0[1];
+      it.h[2] = t0[2];
+      if (it.mass && ss.cross)
+      {
+         const double *t1 = r.cs + ((size_t)it.q * P.nd + it.k) * 3;
+         it.h[3] = t1[0];
+         it.h[4] = t1[1];
+         it.h[5] = t1[2];
+      }
+   }
+   return it;
+}
+
+// Staging rows of one item: A (shape rows), CS (symmetric coefficients),
+// CP (cross coefficients, mass rows with the cross terms only).
+__device__ __forceinline__ void sym_form_item(const LocalParams &P, const double *ec,
+                                              const SymShape &ss, const double *qcoef,
+                                              const SymItem &it, double *stg)
+{
+   constexpr int RN = SYM_ROWS * SYM_NP;
+   double *A = stg, *CSr = stg + RN, *CSi = stg + 2 * RN, *CPr = stg + 3 * RN, *CPi = stg + 4 * RN;
+   const int k = it.k, r0 = it.row0;
+   if (!it.mass)
+   {
+      const double *qc = qcoef + (size_t)it.q * 4;
+      double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+      if (it.live) { piola_c(ec, it.h[0], it.h[1], it.h[2], c0, c1, c2); }
+      const double war = it.valid ? qc[0] : 0.0, wai = it.valid ? qc[1] : 0.0;
+      const double wbr = it.valid ? qc[2] : 0.0, wbi = it.valid ? qc[3] : 0.0;
+      A[(r0 + 0) * SYM_NP + k] = c0;
+      A[(r0 + 1) * SYM_NP + k] = c1;
+      A[(r0 + 2) * SYM_NP + k] = c2;
+      CSr[(r0 + 0) * SYM_NP + k] = war * c0;
+      CSi[(r0 + 0) * SYM_NP + k] = wai * c0;
+      CSr[(r0 + 1) * SYM_NP + k] = war * c1;
+      CSi[(r0 + 1) * SYM_NP + k] = wai * c1;
+      CSr[(r0 + 2) * SYM_NP + k] = wbr * c2;
+      CSi[(r0 + 2) * SYM_NP + k] = wbi * c2;
+      return;
+   }
+   const double *qc = qcoef + (size_t)(P.rc.nq + it.q) * 4;
+   double v0 = 0.0, v1 = 0.0, v2 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+   if (it.live)
+   {
+      piola_v(ec, it.h[0], it.h[1], it.h[2], v0, v1, v2);
+      if (ss.cross) { piola_c(ec, it.h[3], it.h[4], it.h[5], c0, c1, c2); }
+   }
+   const double2 wa = it.valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
+   const double2 wb = it.valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
+   const double *mc = ec + 18;   // m0r m0i m1r m1i m2r m2i kxy c0x c0y c1x c1y
+   const double2 S0 = cmul(wb, make_double2(mc[0] * v0 - mc[6] * v1, mc[1] * v0));
+   const double2 S1 = cmul(wb, make_double2(mc[2] * v1 - mc[6] * v0, mc[3] * v1));
+   const double2 S2 = cmul(wa, make_double2(mc[4] * v2, mc[5] * v2));
+   A[(r0 + 0) * SYM_NP + k] = v0;
+   A[(r0 + 1) * SYM_NP + k] = v1;
+   A[(r0 + 2) * SYM_NP + k] = v2;
+   CSr[(r0 + 0) * SYM_NP + k] = S0.x;
+   CSi[(r0 + 0) * SYM_NP + k] = S0.y;
+   CSr[(r0 + 1) * SYM_NP + k] = S1.x;
+   CSi[(r0 + 1) * SYM_NP + k] = S1.y;
+   CSr[(r0 + 2) * SYM_NP + k] = S2.x;
+   CSi[(r0 + 2) * SYM_NP + k] = S2.y;
+   if (ss.cross)
+   {
+      // the c-terms of C0..C2 of mass_coef: X c
+      const double2 X0 = cmul(wb, make_double2(0.0, mc[8] * c2));
+      const double2 X1 = cmul(wb, make_double2(0.0, -mc[7] * c2));
+      const double2 X2 = cmul(wa, make_double2(0.0, -mc[8] * c0 + mc[7] * c1));
+      CPr[(r0 + 0) * SYM_NP + k] = X0.x;
+      CPi[(r0 + 0) * SYM_NP + k] = X0.y;
+      CPr[(r0 + 1) * SYM_NP + k] = X1.x;
+      CPi[(r0 + 1) * SYM_NP + k] = X1.y;
+      CPr[(r0 + 2) * SYM_NP + k] = X2.x;
+      CPi[(r0 + 2) * SYM_NP + k] = X2.y;
+   }
+}
+
+// Transpose of a 16 x 16 MFMA output tile (lane l = m + 16 g holds rows
+// 4v + g of column m in register v) through a per-wavefront LDS scratch of
+// 16 x 17 doubles.
+__device__ __forceinline__ dbl4 sym_transpose(const dbl4 x, double *tr, int g4, int m16)
+{
+#pragma unroll
+   for (int v = 0; v < 4; ++v) { tr[(4 * v + g4) * 17 + m16] = x[v]; }
+   wave_sync_lds();
+   dbl4 y;
+#pragma unroll
+   for (int v = 0; v < 4; ++v) { y[v] = tr[m16 * 17 + 4 * v + g4]; }
+   wave_sync_lds();
+   return y;
+}
+
+// Dual face transform (columns, then rows; solver/asm_gpu.inc:221-251) and
+// store of output tile (I, J) of element e.
+__device__ __forceinline__ void sym_tile_out(const LocalParams &P, dbl4 (&t)[2], int e, int io,
+                                             int I, int J, int g4, int m16)
+{
+   const int nd = P.nd, p = P.p;
+   if (P.apply_te && p >= 2)
+   {
+      const int foff = 6 * p, ntd = p * (p - 1);
+      const int *fo = P.fo + (size_t)e * 4;
+      {
+         const int col = 16 * J + m16;
+         double cs = 1.0, cp = 0.0;
+         if (col >= foff && col < foff + 4 * ntd)
+         {
+            const double *Ti = P.TInv + 4 * fo[(col - foff) / ntd];
+            const bool even = ((col - foff) & 1) == 0;
+            cs = even ? Ti[0] : Ti[3];
+            cp = even ? Ti[1] : Ti[2];
+         }
+#pragma unroll
+         for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+               const double x = t[c][v];
+               const double y = __shfl_xor(x, 1);
+               t[c][v] = cs * x + cp * y;
+            }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+      {
+         const int row = 16 * I + g4 + 4 * v;
+         double cs = 1.0, cp = 0.0;
+         if (row >= foff && row < foff + 4 * ntd)
+         {
+            const double *Ti = P.TInv + 4 * fo[(row - foff) / ntd];
+            const bool even = ((row - foff) & 1) == 0;
+            cs = even ? Ti[0] : Ti[3];
+            cp = even ? Ti[1] : Ti[2];
+         }
+#pragma unroll
+         for (int c = 0; c < 2; ++c)
+         {
+            const double x = t[c][v];
+            const double y = __shfl_xor(x, 16);
+            t[c][v] = cs * x + cp * y;
+         }
+      }
+   }
+   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * P.lda;
+   const bool lines = (P.lda & 7) == 0;
+   const int col = 16 * J + m16;
+#pragma unroll
+   for (int v = 0; v < 4; ++v)
+   {
+      const int row = 16 * I + g4 + 4 * v;
+      if (row < nd && col < P.lda)
+      {
+         if (lines)
+         {
+            __builtin_nontemporal_store(dbl2v{t[0][v], t[1][v]},
+                                        reinterpret_cast<dbl2v *>(Ae) + (size_t)row * P.lda + col);
+         }
+         else if (col < nd)
+         {
+            Ae[(size_t)row * P.lda + col] = make_double2(t[0][v], t[1][v]);
+         }
+      }
+   }
+}
+
+// One element per workgroup of 3 wavefronts.  NBUF = 2: double-buffered
+// staging as in k_local_mfma_pipe (the table loads of chunk c+1 are issued
+// before the MFMAs of chunk c, its rows formed after them); NBUF = 1: one
+// staging buffer (half the LDS, more workgroups per CU), the next chunk
+// formed between two barriers.
+template <int NBUF, int WAVES_PER_EU>
+__global__ void __launch_bounds__(SYM_THREADS)
+__attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
+k_local_mfma_sym(LocalParams P, int nqt)
+{
+   constexpr int RN = SYM_ROWS * SYM_NP;
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int tid = threadIdx.x;
+   const int io = blockIdx.x;   // grid = nout
+   const int e = P.elist ? P.elist[io] : io;
+
+   extern __shared__ double smem[];
+   double *ec = smem;
+   double *qcoef = ec + EC_SIZE;
+   double *buf = qcoef + nqt * 4;
+   elem_setup(P, e, nqt, tid, SYM_THREADS, qcoef, ec);
+   const SymShape ss = sym_shape(P);
+   __syncthreads();
+
+   // tiles of this wavefront: the pair (ti, tj) and the diagonal td
+   const int ti = wave == 2 ? 1 : 0;
+   const int tj = wave == 0 ? 1 : 2;
+   const int td = 2 - wave;
+   dbl4 sij[2], pij[2], pji[2], sdd[2], pdd[2];
+#pragma unroll
+   for (int c = 0; c < 2; ++c)
+   {
+      sij[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+      pij[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+      pji[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+      sdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+      pdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+   }
+   const int g4 = lane >> 4, m16 = lane & 15;
+   const int nch = ss.nch_cc + ss.nch_m;
+   if (nch > 0) { sym_form_item(P, ec, ss, qcoef, sym_load_item(P, ss, 0, tid), buf); }
+   __syncthreads();
+   for (int ch = 0; ch < nch; ++ch)
+   {
+      const double *A = buf + (NBUF == 2 ? (ch & 1) * SYM_STG : 0);
+      const double *CSr = A + RN, *CSi = A + 2 * RN, *CPr = A + 3 * RN, *CPi = A + 4 * RN;
+      const bool more = ch + 1 < nch;
+      const SymItem nxt = sym_load_item(P, ss, more ? ch + 1 : ch, tid);
+      const bool cross = ss.cross && ch >= ss.nch_cc;   // wave-uniform
+#pragma unroll
+      for (int ks = 0; ks < SYM_ROWS / 4; ++ks)
+      {
+         const int o = (ks * 4 + g4) * SYM_NP + m16;
+         const double ai = A[o + 16 * ti], aj = A[o + 16 * tj], ad = A[o + 16 * td];
+         const double sjr = CSr[o + 16 * tj], sji = CSi[o + 16 * tj];
+         const double sdr = CSr[o + 16 * td], sdi = CSi[o + 16 * td];
+         sij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sjr, sij[0], 0, 0, 0);
+         sij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sji, sij[1], 0, 0, 0);
+         sdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdr, sdd[0], 0, 0, 0);
+         sdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdi, sdd[1], 0, 0, 0);
+         if (cross)
+         {
+            const double pjr = CPr[o + 16 * tj], pjq = CPi[o + 16 * tj];
+            const double pir = CPr[o + 16 * ti], piq = CPi[o + 16 * ti];
+            const double pdr = CPr[o + 16 * td], pdq = CPi[o + 16 * td];
+            pij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, pjr, pij[0], 0, 0, 0);
+            pij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, pjq, pij[1], 0, 0, 0);
+            pji[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj, pir, pji[0], 0, 0, 0);
+            pji[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj, piq, pji[1], 0, 0, 0);
+            pdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdr, pdd[0], 0, 0, 0);
+            pdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdq, pdd[1], 0, 0, 0);
+         }
+      }
+      if (NBUF == 1) { __syncthreads(); }
+      if (more) { sym_form_item(P, ec, ss, qcoef, nxt, buf + (NBUF == 2 ? ((ch + 1) & 1) * SYM_STG : 0)); }
+      __syncthreads();
+   }
+
+   // the staging is free now: per-wavefront transpose scratch
+   double *tr = buf + wave * 16 * 17;
+   dbl4 t1[2], t2[2], t3[2];
+#pragma unroll
+   for (int c = 0; c < 2; ++c)
+   {
+      const dbl4 sijT = sym_transpose(sij[c], tr, g4, m16);
+      const dbl4 pijT = sym_transpose(pij[c], tr, g4, m16);
+      const dbl4 pjiT = sym_transpose(pji[c], tr, g4, m16);
+      const dbl4 pddT = sym_transpose(pdd[c], tr, g4, m16);
+      t1[c] = sij[c] + pij[c] - pjiT;
+      t2[c] = sijT + pji[c] - pijT;
+      t3[c] = sdd[c] + pdd[c] - pddT;
+   }
+   sym_tile_out(P, t1, e, io, ti, tj, g4, m16);
+   sym_tile_out(P, t2, e, io, tj, ti, g4, m16);
+   sym_tile_out(P, t3, e, io, td, td, g4, m16);
+}
+
+}  // namespace lfasm

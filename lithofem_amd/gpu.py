@@ -28,7 +28,7 @@ E_ARG, E_HIP, E_FORCED, E_UNSUPPORTED, E_STATE, E_PATTERN = -1, -2, -3, -4, -5, 
 
 # kernel selector of lfasm_compute_local_kernel
 KERNEL_AUTO, KERNEL_VALU = 0, 1
-N_MFMA_VARIANTS = 4
+N_MFMA_VARIANTS = 6
 
 
 def kernel_mfma(variant: int = 0) -> int:
