@@ -220,6 +220,18 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in (r.stdout + r.stderr)
 
 
+def test_bench_reads_the_committed_traffic_counters():
+    """roofline.traffic of the default run comes from the counter summary
+    under profiles/; its kernel names must be the ones bench.py reports."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    t = bench._measured_traffic()
+    assert set(t) == {"k_scatter_rows", "k_tensor_coef+k_local_tensor21"}
+    # HBM bytes per launch near the algorithmic bytes (4.55 GB, 1.91 GB)
+    assert 4.5e9 < t["k_scatter_rows"] < 6.0e9
+    assert 1.9e9 < t["k_tensor_coef+k_local_tensor21"] < 2.5e9
+
+
 def test_matrixmarket_round_trip(tmp_path):
     """The oracle's eliminated system and an RHS written in the reference's
     MatrixMarket format read back bit for bit (17 significant digits)."""
