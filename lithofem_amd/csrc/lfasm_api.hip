@@ -94,6 +94,8 @@ struct lfasm_handle {
    int *Ip = nullptr, *Jc = nullptr, *rc_ptr = nullptr, *rc_list = nullptr, *bad = nullptr;
    uint16_t *slot = nullptr;
    int sld = 0;                       // slot row stride (entries)
+   int *rsoff = nullptr;              // per row: start of its slot block in rslot
+   uint16_t *rslot = nullptr;         // slot rows in each row's contributor order
    unsigned char *ess = nullptr, *eflag = nullptr;
    uint64_t *emask = nullptr;         // nd <= 64: per element sign and essential bit masks
    double2 *vals = nullptr;
@@ -220,6 +222,7 @@ void release_global(lfasm_handle *h)
    dfree(h->rc_enc);
    h->fused_ok = false;
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
+   dfree(h->rsoff); dfree(h->rslot);
    dfree(h->emask);
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
    dfree(h->rhs); dfree(h->d_deps); dfree(h->d_depsT); dfree(h->vfix); dfree(h->vcopied);
@@ -819,14 +822,14 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.emask = h->emask;
    S.rc_ptr = h->rc_ptr;
    S.rc_list = h->rc_list;
-   S.slot = h->slot;
-   S.sld = h->sld;
    S.is_ess = h->ess;
    S.Ae = h->Ae;
    S.lda = h->lda;
    S.vals = h->vals;
    S.eliminate = eliminate ? 1 : 0;
    S.maxrow = h->maxrow;
+   S.rsoff = h->rsoff;
+   S.rslot = h->rslot;
    const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
    LF_TRY(hipFuncSetAttribute((const void *)k_scatter_rows,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
@@ -1261,6 +1264,28 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
    {
       release_global(h);
       return h->fail(LFASM_E_PATTERN, "an element (row, col) pair is missing from the CSR pattern");
+   }
+   {
+      // slot rows regrouped per CSR row, in the row's contributor order: the
+      // scatter reads a row's slot rows from one contiguous block
+      std::vector<int> rsoff((size_t)n + 1, 0);
+      for (int r = 0; r < n; ++r)
+      {
+         const int64_t len = (int64_t)(rc_ptr[(size_t)r + 1] - rc_ptr[(size_t)r]) * (int64_t)nd;
+         const int64_t next = rsoff[(size_t)r] + len;
+         if (next > INT32_MAX)
+         {
+            release_global(h);
+            return h->fail(LFASM_E_UNSUPPORTED, "slot blocks exceed 2^31 entries");
+         }
+         rsoff[(size_t)r + 1] = (int)next;
+      }
+      LF_HIP(upload(h, h->rsoff, rsoff.data(), rsoff.size()));
+      LF_HIP(hipMalloc((void **)&h->rslot, std::max<size_t>(rsoff[(size_t)n], 1) * sizeof(uint16_t)));
+      hipLaunchKernelGGL(k_row_slots, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0, h->stream, n,
+                         (int)nd, h->sld, (const int *)h->rc_ptr, (const int *)h->rc_list,
+                         (const uint16_t *)h->slot, (const int *)h->rsoff, h->rslot);
+      LF_HIP(hipGetLastError());
    }
    if (h->coef && h->p <= 3)
    {
