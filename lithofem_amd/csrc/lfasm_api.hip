@@ -95,7 +95,8 @@ struct lfasm_handle {
    uint16_t *slot = nullptr;
    int sld = 0;                       // slot row stride (entries)
    int *rsoff = nullptr;              // per row: start of its slot block in rslot
-   uint16_t *rslot = nullptr;         // slot rows in each row's contributor order
+   void *rslot = nullptr;             // slot rows in each row's contributor order
+   bool rslot8 = false;               // rslot entries are bytes (every row <= 256 entries)
    unsigned char *ess = nullptr, *eflag = nullptr;
    uint64_t *emask = nullptr;         // nd <= 64: per element sign and essential bit masks
    double2 *vals = nullptr;
@@ -831,10 +832,19 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.rsoff = h->rsoff;
    S.rslot = h->rslot;
    const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
-   LF_TRY(hipFuncSetAttribute((const void *)k_scatter_rows,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+   const void *kern = h->rslot8 ? (const void *)k_scatter_rows<uint8_t> : (const void *)k_scatter_rows<uint16_t>;
+   LF_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
    if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
-   hipLaunchKernelGGL(k_scatter_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem, h->stream, S);
+   if (h->rslot8)
+   {
+      hipLaunchKernelGGL(k_scatter_rows<uint8_t>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
+                         h->stream, S);
+   }
+   else
+   {
+      hipLaunchKernelGGL(k_scatter_rows<uint16_t>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
+                         h->stream, S);
+   }
    LF_TRY(hipGetLastError());
    if (timed) { LF_TRY(ev_end(h, T_SCATTER)); }
    return hipSuccess;
@@ -1281,10 +1291,24 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
          rsoff[(size_t)r + 1] = (int)next;
       }
       LF_HIP(upload(h, h->rsoff, rsoff.data(), rsoff.size()));
-      LF_HIP(hipMalloc((void **)&h->rslot, std::max<size_t>(rsoff[(size_t)n], 1) * sizeof(uint16_t)));
-      hipLaunchKernelGGL(k_row_slots, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0, h->stream, n,
-                         (int)nd, h->sld, (const int *)h->rc_ptr, (const int *)h->rc_list,
-                         (const uint16_t *)h->slot, (const int *)h->rsoff, h->rslot);
+      // byte entries when every row has at most 256 entries (half the slot bytes)
+      h->rslot8 = maxrow <= 256;
+      const size_t esz = h->rslot8 ? 1 : 2;
+      LF_HIP(hipMalloc((void **)&h->rslot, std::max<size_t>(rsoff[(size_t)n], 1) * esz));
+      if (h->rslot8)
+      {
+         hipLaunchKernelGGL(k_row_slots<uint8_t>, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0,
+                            h->stream, n, (int)nd, h->sld, (const int *)h->rc_ptr,
+                            (const int *)h->rc_list, (const uint16_t *)h->slot,
+                            (const int *)h->rsoff, (uint8_t *)h->rslot);
+      }
+      else
+      {
+         hipLaunchKernelGGL(k_row_slots<uint16_t>, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0,
+                            h->stream, n, (int)nd, h->sld, (const int *)h->rc_ptr,
+                            (const int *)h->rc_list, (const uint16_t *)h->slot,
+                            (const int *)h->rsoff, (uint16_t *)h->rslot);
+      }
       LF_HIP(hipGetLastError());
    }
    if (h->coef && h->p <= 3)

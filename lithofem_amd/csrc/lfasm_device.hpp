@@ -849,23 +849,25 @@ __global__ void k_slot_map(int ne, int nd, int sld, const int *__restrict__ dof,
 
 // Slot rows regrouped per CSR row, in the row's contributor order
 // (rc_list): block of row r = rslot[rsoff[r] ..], contributor c - c0 at
-// (c - c0) * nd.  One wavefront per row.
+// (c - c0) * nd.  T = uint8_t when every row has at most 256 entries.  One
+// wavefront per row.
+template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_row_slots(int n, int nd, int sld,
                                                      const int *__restrict__ rc_ptr,
                                                      const int *__restrict__ rc_list,
                                                      const uint16_t *__restrict__ slot,
                                                      const int *__restrict__ rsoff,
-                                                     uint16_t *__restrict__ rslot)
+                                                     T *__restrict__ rslot)
 {
    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
    if (r >= n) { return; }
    const int c0 = rc_ptr[r], c1 = rc_ptr[r + 1];
-   uint16_t *dst = rslot + rsoff[r];
+   T *dst = rslot + rsoff[r];
    for (int c = c0; c < c1; ++c)
    {
       const int ea = rc_list[c];
-      for (int b = lane; b < nd; b += 64) { dst[(size_t)(c - c0) * nd + b] = slot[(size_t)ea * sld + b]; }
+      for (int b = lane; b < nd; b += 64) { dst[(size_t)(c - c0) * nd + b] = (T)slot[(size_t)ea * sld + b]; }
    }
 }
 
@@ -889,7 +891,8 @@ struct ScatterParams {
    int eliminate;
    int maxrow;
    const int *rsoff;            // per row: start of its slot block in rslot
-   const uint16_t *rslot;       // the row's contributors' slot rows, in contributor order
+   const void *rslot;           // the row's contributors' slot rows, in contributor order
+                                // (uint8_t or uint16_t entries: the kernel's template type)
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -908,9 +911,9 @@ constexpr int SCATTER_CG = 4;   // 6 and 8 measured slower at p = 3
 // buffer: every load of the group is issued before the first use, with no
 // branch between them, so they are in flight together; the accumulation
 // then runs in ascending element order (deterministic sums).
-template <int M>
+template <int M, typename T>
 __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *buf, int cg, int lane,
-                                              unsigned ess_bit, const uint16_t *sblk)
+                                              unsigned ess_bit, const T *sblk)
 {
    const int nd = S.nd;
    int ea[M];   // e*nd + a
@@ -968,18 +971,19 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
 }
 
 // scatter_group<m> for a group of m <= M contributors (m is wave-uniform)
-template <int M>
+template <int M, typename T>
 __device__ __forceinline__ void scatter_dispatch(const ScatterParams &S, double2 *buf, int cg, int m,
-                                                 int lane, unsigned ess_bit, const uint16_t *sblk)
+                                                 int lane, unsigned ess_bit, const T *sblk)
 {
-   if constexpr (M == 1) { scatter_group<1>(S, buf, cg, lane, ess_bit, sblk); }
+   if constexpr (M == 1) { scatter_group<1, T>(S, buf, cg, lane, ess_bit, sblk); }
    else
    {
-      if (m == M) { scatter_group<M>(S, buf, cg, lane, ess_bit, sblk); }
-      else { scatter_dispatch<M - 1>(S, buf, cg, m, lane, ess_bit, sblk); }
+      if (m == M) { scatter_group<M, T>(S, buf, cg, lane, ess_bit, sblk); }
+      else { scatter_dispatch<M - 1, T>(S, buf, cg, m, lane, ess_bit, sblk); }
    }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
 {
    extern __shared__ double2 rowbuf[];
@@ -997,7 +1001,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
    if (!row_ess)
    {
       const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
-      const uint16_t *sblk = S.rslot + S.rsoff[r];
+      const T *sblk = reinterpret_cast<const T *>(S.rslot) + S.rsoff[r];
       for (int cg = c0; cg < c1; cg += SCATTER_CG)
       {
          scatter_dispatch<SCATTER_CG>(S, buf, cg, min(SCATTER_CG, c1 - cg), lane, ess_bit,
