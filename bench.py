@@ -236,7 +236,9 @@ def main() -> int:
         f"{download_ms:.1f} ms")
 
     W = canonical_flops(ad.p, ad.nd, ad.rule_cc.nq, ad.rule_mass.nq, pb.has_kpar)
-    q_sc = scatter_bytes(ad.ne, ad.nd, ad.height, len(J))
+    # the library's slot entries are bytes when no CSR row exceeds 256 entries
+    q_sc = scatter_bytes(ad.ne, ad.nd, ad.height, len(J),
+                         1 if int(np.diff(I).max()) <= 256 else 2)
     kernels = {
         "k_scatter_rows": {"ms": ts_avg, "bound": "hbm", "unit": "GB/s",
                            "achieved": q_sc / (ts_avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,

@@ -8,8 +8,9 @@ structural nonzeros, Piola maps, face transform):
     d_cc = 3, z_cc = 3;  d_mass = 6, z_mass = 13 with k_par != 0, else 3 and 3.
 
 ``scatter_bytes`` is the compulsory HBM traffic of the materialised-Ae
-scatter: read Ae (16 B) and its slot (2 B) per local entry, write each CSR
-value once (16 B), plus the row pointers, contributor lists and signs.
+scatter: read Ae (16 B) and its slot (1 B when every CSR row has at most
+256 entries, else 2 B) per local entry, write each CSR value once (16 B),
+plus the row pointers, contributor lists and signs.
 """
 
 from __future__ import annotations
@@ -44,8 +45,8 @@ def tensor_bytes(ne: int, nd: int) -> float:
     return float(ne) * (4 + 9 * 8 * 2 + 8 + 4 + 16 + 16.0 * nd * nd)
 
 
-def scatter_bytes(ne: int, nd: int, n: int, nnz: int) -> float:
-    return float(ne) * nd * nd * (16 + 2) + 16.0 * nnz + 4.0 * (n + 1) + \
+def scatter_bytes(ne: int, nd: int, n: int, nnz: int, slot_bytes: int = 2) -> float:
+    return float(ne) * nd * nd * (16 + slot_bytes) + 16.0 * nnz + 4.0 * (n + 1) + \
         float(ne) * nd * (4 + 4 + 8) + 4.0 * (n + 1)
 
 
