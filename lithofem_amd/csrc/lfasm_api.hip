@@ -95,8 +95,8 @@ struct lfasm_handle {
    uint16_t *slot = nullptr;
    int sld = 0;                       // slot row stride (entries)
    int *rsoff = nullptr;              // per row: start of its slot block in rslot
-   void *rslot = nullptr;             // slot rows in each row's contributor order
-   bool rslot8 = false;               // rslot entries are bytes (every row <= 256 entries)
+   uint8_t *rslot = nullptr;          // byte slot rows in each row's contributor order
+                                      // (built when every row has at most 256 entries)
    unsigned char *ess = nullptr, *eflag = nullptr;
    uint64_t *emask = nullptr;         // nd <= 64: per element sign and essential bit masks
    double2 *vals = nullptr;
@@ -829,20 +829,23 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.vals = h->vals;
    S.eliminate = eliminate ? 1 : 0;
    S.maxrow = h->maxrow;
+   S.slot = h->slot;
+   S.sld = h->sld;
    S.rsoff = h->rsoff;
    S.rslot = h->rslot;
    const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
-   const void *kern = h->rslot8 ? (const void *)k_scatter_rows<uint8_t> : (const void *)k_scatter_rows<uint16_t>;
+   const bool bytes = h->rslot != nullptr;
+   const void *kern = bytes ? (const void *)k_scatter_rows<true> : (const void *)k_scatter_rows<false>;
    LF_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
    if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
-   if (h->rslot8)
+   if (bytes)
    {
-      hipLaunchKernelGGL(k_scatter_rows<uint8_t>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
+      hipLaunchKernelGGL(k_scatter_rows<true>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
                          h->stream, S);
    }
    else
    {
-      hipLaunchKernelGGL(k_scatter_rows<uint16_t>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
+      hipLaunchKernelGGL(k_scatter_rows<false>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
                          h->stream, S);
    }
    LF_TRY(hipGetLastError());
@@ -1275,9 +1278,11 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
       release_global(h);
       return h->fail(LFASM_E_PATTERN, "an element (row, col) pair is missing from the CSR pattern");
    }
+   if (maxrow <= 256)
    {
-      // slot rows regrouped per CSR row, in the row's contributor order: the
-      // scatter reads a row's slot rows from one contiguous block
+      // byte slot rows regrouped per CSR row, in the row's contributor order
+      // (every row has at most 256 entries): the scatter reads a row's slot
+      // rows from one contiguous block
       std::vector<int> rsoff((size_t)n + 1, 0);
       for (int r = 0; r < n; ++r)
       {
@@ -1291,24 +1296,10 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
          rsoff[(size_t)r + 1] = (int)next;
       }
       LF_HIP(upload(h, h->rsoff, rsoff.data(), rsoff.size()));
-      // byte entries when every row has at most 256 entries (half the slot bytes)
-      h->rslot8 = maxrow <= 256;
-      const size_t esz = h->rslot8 ? 1 : 2;
-      LF_HIP(hipMalloc((void **)&h->rslot, std::max<size_t>(rsoff[(size_t)n], 1) * esz));
-      if (h->rslot8)
-      {
-         hipLaunchKernelGGL(k_row_slots<uint8_t>, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0,
-                            h->stream, n, (int)nd, h->sld, (const int *)h->rc_ptr,
-                            (const int *)h->rc_list, (const uint16_t *)h->slot,
-                            (const int *)h->rsoff, (uint8_t *)h->rslot);
-      }
-      else
-      {
-         hipLaunchKernelGGL(k_row_slots<uint16_t>, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0,
-                            h->stream, n, (int)nd, h->sld, (const int *)h->rc_ptr,
-                            (const int *)h->rc_list, (const uint16_t *)h->slot,
-                            (const int *)h->rsoff, (uint16_t *)h->rslot);
-      }
+      LF_HIP(hipMalloc((void **)&h->rslot, std::max<size_t>(rsoff[(size_t)n], 1)));
+      hipLaunchKernelGGL(k_row_slots, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0, h->stream, n,
+                         (int)nd, h->sld, (const int *)h->rc_ptr, (const int *)h->rc_list,
+                         (const uint16_t *)h->slot, (const int *)h->rsoff, h->rslot);
       LF_HIP(hipGetLastError());
    }
    if (h->coef && h->p <= 3)

@@ -847,27 +847,26 @@ __global__ void k_slot_map(int ne, int nd, int sld, const int *__restrict__ dof,
    }
 }
 
-// Slot rows regrouped per CSR row, in the row's contributor order
-// (rc_list): block of row r = rslot[rsoff[r] ..], contributor c - c0 at
-// (c - c0) * nd.  T = uint8_t when every row has at most 256 entries.  One
-// wavefront per row.
-template <typename T>
+// Slot rows regrouped per CSR row as bytes (every row has at most 256
+// entries), in the row's contributor order (rc_list): block of row r =
+// rslot[rsoff[r] ..], contributor c - c0 at (c - c0) * nd.  One wavefront
+// per row.
 __global__ void __launch_bounds__(BLOCK) k_row_slots(int n, int nd, int sld,
                                                      const int *__restrict__ rc_ptr,
                                                      const int *__restrict__ rc_list,
                                                      const uint16_t *__restrict__ slot,
                                                      const int *__restrict__ rsoff,
-                                                     T *__restrict__ rslot)
+                                                     uint8_t *__restrict__ rslot)
 {
    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
    if (r >= n) { return; }
    const int c0 = rc_ptr[r], c1 = rc_ptr[r + 1];
-   T *dst = rslot + rsoff[r];
+   uint8_t *dst = rslot + rsoff[r];
    for (int c = c0; c < c1; ++c)
    {
       const int ea = rc_list[c];
-      for (int b = lane; b < nd; b += 64) { dst[(size_t)(c - c0) * nd + b] = (T)slot[(size_t)ea * sld + b]; }
+      for (int b = lane; b < nd; b += 64) { dst[(size_t)(c - c0) * nd + b] = (uint8_t)slot[(size_t)ea * sld + b]; }
    }
 }
 
@@ -890,9 +889,11 @@ struct ScatterParams {
    double2 *vals;
    int eliminate;
    int maxrow;
+   const uint16_t *slot;        // element-major slot map (rows of sld entries)
+   int sld;
    const int *rsoff;            // per row: start of its slot block in rslot
-   const void *rslot;           // the row's contributors' slot rows, in contributor order
-                                // (uint8_t or uint16_t entries: the kernel's template type)
+   const uint8_t *rslot;        // the row's contributors' slot rows as bytes, in
+                                // contributor order (rows of at most 256 entries)
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -911,9 +912,9 @@ constexpr int SCATTER_CG = 4;   // 6 and 8 measured slower at p = 3
 // buffer: every load of the group is issued before the first use, with no
 // branch between them, so they are in flight together; the accumulation
 // then runs in ascending element order (deterministic sums).
-template <int M, typename T>
+template <int M, bool BYTE_SLOTS>
 __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *buf, int cg, int lane,
-                                              unsigned ess_bit, const T *sblk)
+                                              unsigned ess_bit, const uint8_t *sblk)
 {
    const int nd = S.nd;
    int ea[M];   // e*nd + a
@@ -932,11 +933,16 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
          x[u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2v *>(S.Ae) +
                                            (size_t)ea[u] * S.lda + bc);
       }
-      // the group's slot rows, contiguous in the row's block (read from the
-      // element-major slot map instead, where a row's slot rows lie apart:
-      // 0.97 against 0.915 ms at 1.35M DOF)
+      // byte slots: the group's slot rows, contiguous in the row's block
+      // (the element-major 16-bit map, where a row's slot rows lie apart,
+      // ran 0.97 against 0.874 ms at 1.35M DOF, p = 3; at p = 4, rows above
+      // 256 entries, a per-row 16-bit copy was slower than the map)
 #pragma unroll
-      for (int u = 0; u < M; ++u) { sl[u] = sblk[u * nd + bc]; }
+      for (int u = 0; u < M; ++u)
+      {
+         if constexpr (BYTE_SLOTS) { sl[u] = sblk[u * nd + bc]; }
+         else { sl[u] = S.slot[(size_t)ea[u] * S.sld + bc]; }
+      }
 #pragma unroll
       for (int u = 0; u < M; ++u)
       {
@@ -971,19 +977,19 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
 }
 
 // scatter_group<m> for a group of m <= M contributors (m is wave-uniform)
-template <int M, typename T>
+template <int M, bool BYTE_SLOTS>
 __device__ __forceinline__ void scatter_dispatch(const ScatterParams &S, double2 *buf, int cg, int m,
-                                                 int lane, unsigned ess_bit, const T *sblk)
+                                                 int lane, unsigned ess_bit, const uint8_t *sblk)
 {
-   if constexpr (M == 1) { scatter_group<1, T>(S, buf, cg, lane, ess_bit, sblk); }
+   if constexpr (M == 1) { scatter_group<1, BYTE_SLOTS>(S, buf, cg, lane, ess_bit, sblk); }
    else
    {
-      if (m == M) { scatter_group<M, T>(S, buf, cg, lane, ess_bit, sblk); }
-      else { scatter_dispatch<M - 1, T>(S, buf, cg, m, lane, ess_bit, sblk); }
+      if (m == M) { scatter_group<M, BYTE_SLOTS>(S, buf, cg, lane, ess_bit, sblk); }
+      else { scatter_dispatch<M - 1, BYTE_SLOTS>(S, buf, cg, m, lane, ess_bit, sblk); }
    }
 }
 
-template <typename T>
+template <bool BYTE_SLOTS>
 __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
 {
    extern __shared__ double2 rowbuf[];
@@ -1001,11 +1007,11 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
    if (!row_ess)
    {
       const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
-      const T *sblk = reinterpret_cast<const T *>(S.rslot) + S.rsoff[r];
+      const uint8_t *sblk = BYTE_SLOTS ? S.rslot + S.rsoff[r] : nullptr;
       for (int cg = c0; cg < c1; cg += SCATTER_CG)
       {
-         scatter_dispatch<SCATTER_CG>(S, buf, cg, min(SCATTER_CG, c1 - cg), lane, ess_bit,
-                                      sblk + (size_t)(cg - c0) * S.nd);
+         scatter_dispatch<SCATTER_CG, BYTE_SLOTS>(S, buf, cg, min(SCATTER_CG, c1 - cg), lane, ess_bit,
+                                                  BYTE_SLOTS ? sblk + (size_t)(cg - c0) * S.nd : nullptr);
       }
    }
    else
