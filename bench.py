@@ -296,6 +296,8 @@ def main() -> int:
                    "elements_tensor_path": nconst, "elements_quadrature": nquad,
                    "element_matrices_quadrature_only_ms": quad_only_ms,
                    "quadrature_kernel": tuned["best"] if tuned else args.kernel,
+                   "quadrature_kernel_name": _quad_kernel_name(
+                       tuned["best"] if tuned else args.kernel, ad.nd),
                    "autotune_ms": {str(k): v for k, v in tuned["ms"].items()} if tuned else None,
                    "canonical_flops_per_element": W,
                    "canonical_element_TFLOPs": W * ad.ne / (tl_avg * 1e-3) / 1e12,
@@ -327,6 +329,17 @@ def main() -> int:
     g.close()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _quad_kernel_name(selector: int, nd: int) -> str:
+    """Kernel behind a selector of lfasm_compute_local_kernel for 32 < nd <= 48
+    with all five integrators (the launch table of lfasm_api.hip)."""
+    if not 32 < nd <= 48:
+        return f"selector {selector}"
+    return {1: "k_local_generic<4, 512> + k_te", 2: "k_local_mfma_sym<1, 3>",
+            3: "k_local_mfma_pipe<3, 2, 2>", 4: "k_local_mfma_pipe<3, 1, 4>",
+            5: "k_local_mfma<3, 1, 1, 2>", 6: "k_local_mfma_sym<2, 3>",
+            7: "k_local_mfma_pipe<3, 1, 2>"}.get(selector, f"selector {selector}")
 
 
 def _measured_traffic() -> dict:
