@@ -267,16 +267,21 @@ int lfasm_set_incident(lfasm_handle *h, const lfasm_incident *inc);
 /* RHS of every incident field (ScatterSource, solver/problem_common.inc:
  * 238-266): b_k = int k0^2 deps E(z) . v_k on the mass rule, dual face
  * transform, sign scatter (ascending element order); essential rows zeroed
- * with `eliminate` (solver/lithofem_solve.cpp:732-742).  Device result
- * nrhs x n complex. */
+ * with `eliminate` (solver/lithofem_solve.cpp:732-742).  deps is the
+ * region's eps - eps_bg in source regions (0 elsewhere), as the reference;
+ * with per-element eps (lfasm_set_eps_elem) it is eps_e - eps_bg in every
+ * region, eps_bg = region eps - region deps, so an element whose eps
+ * differs from its slab background is a source wherever it lies.  Device
+ * result nrhs x n complex. */
 int lfasm_assemble_rhs(lfasm_handle *h, int eliminate);
 int lfasm_download_rhs(lfasm_handle *h, double *re, double *im);
 
 /* Adjoint sensitivity per parameter element (sorted unique element ids):
  * s_e = lam^T (dA/deps_e u - db/deps_e) with lam, u global complex vectors
- * (n interleaved, host); the RHS term uses incident field rhs_col of the
- * element's problem in source regions (rhs_col < 0: matrix term only).
- * s_out: nparam interleaved complex. */
+ * (n interleaved, host); db/deps_e is the element RHS of incident field
+ * rhs_col of the element's problem at unit source strength, in every
+ * region (eps_e enters eps_e - eps_bg linearly; rhs_col < 0: matrix term
+ * only).  s_out: nparam interleaved complex. */
 int lfasm_eps_sensitivity(lfasm_handle *h, int nparam, const int *elems, const double *lam,
                           const double *u, int rhs_col, double *s_out);
 

@@ -94,8 +94,11 @@ def locate(ad: AsmData, pts: np.ndarray, cand=None):
 
 def sampling_matrix(ad: AsmData, pts: np.ndarray, periodic=None) -> sp.csr_matrix:
     """S (3 npts x n): S @ u = field components at the points.  `periodic`
-    = (Lx, Ly) wraps points into the cell first."""
+    = (Lx, Ly) wraps points into the cell [0, Lx) x [0, Ly) first."""
     pts = np.array(pts, dtype=float)
+    if periodic is not None:
+        pts[:, 0] = np.mod(pts[:, 0], periodic[0])
+        pts[:, 1] = np.mod(pts[:, 1], periodic[1])
     e, xi = locate(ad, pts)
     el = nd_tet(ad.p)
     nd, p = ad.nd, ad.p
@@ -126,11 +129,19 @@ def sampling_matrix(ad: AsmData, pts: np.ndarray, periodic=None) -> sp.csr_matri
     return S
 
 
-def plane_points(lx, ly, z, nx, ny):
-    x = (np.arange(nx) + 0.5) * lx / nx
-    y = (np.arange(ny) + 0.5) * ly / ny
+# the reference's plane sampler moves every sample point by a small fixed
+# offset so that no point lies on an element face or edge and the element
+# that evaluates it is deterministic (solver/lithofem_solve.cpp:961-964)
+PLANE_NUDGE = (3.1e-7, 1.7e-7, -4.9e-7)
+
+
+def plane_points(lx, ly, z, nx, ny, nudge=PLANE_NUDGE):
+    """Cell-centred nx x ny grid on the plane z (x index first), shifted by
+    `nudge` as the reference's plane sampler does."""
+    x = (np.arange(nx) + 0.5) * lx / nx + nudge[0]
+    y = (np.arange(ny) + 0.5) * ly / ny + nudge[1]
     X, Y = np.meshgrid(x, y, indexing="ij")
-    return np.stack([X.ravel(), Y.ravel(), np.full(X.size, float(z))], 1)
+    return np.stack([X.ravel(), Y.ravel(), np.full(X.size, float(z) + nudge[2])], 1)
 
 
 # ---------------------------------------------------------------- forward model

@@ -59,6 +59,9 @@ struct lfasm_handle {
    int *attr = nullptr, *fo = nullptr, *dof = nullptr;
    double *sign = nullptr;
    double2 *eps_tab = nullptr, *eps_elem = nullptr;
+   // host copies: region eps table (concatenated per problem) and the
+   // per-element eps (empty when the region table is in use)
+   std::vector<double2> eps_tab_host, eps_elem_host;
    DevRule rc, rm;
    PmlDev *pml_tab = nullptr;
    int *pid = nullptr, *eps_off = nullptr;
@@ -119,6 +122,7 @@ struct lfasm_handle {
    SlabWaveDev *waves = nullptr;
    int *region_slab = nullptr, *region_src = nullptr;
    std::vector<int> region_src_host;
+   std::vector<double2> region_deps_host;
    double2 *region_deps = nullptr, *rhs = nullptr;
    // source elements of the incident set (built at the first RHS assembly
    // after lfasm_set_incident) and the per-element RHS staging
@@ -202,6 +206,10 @@ void release_mesh(lfasm_handle *h)
    dfree(h->rhs_src); dfree(h->rhs_index); dfree(h->rhs_bloc);
    h->rhs_nsrc = -1;
    h->inc_nrhs = 0;
+   h->eps_tab_host.clear();
+   h->eps_elem_host.clear();
+   h->region_src_host.clear();
+   h->region_deps_host.clear();
    for (DevRule *r : {&h->rc, &h->rm})
    {
       dfree(r->ip); dfree(r->w); dfree(r->vs); dfree(r->cs);
@@ -950,6 +958,25 @@ int validate_mesh(lfasm_handle *h, const lfasm_mesh *m, int idx)
                         m->height);
       }
    }
+   // the local dofs of an element are distinct global dofs: the row scatter
+   // accumulates one contributor row's entries in one LDS instruction, where
+   // two equal columns would lose a contribution
+   {
+      std::vector<int> stamp((size_t)m->height, -1);
+      for (int e = 0; e < m->ne; ++e)
+      {
+         const int *ed = m->dof + (size_t)e * m->nd;
+         for (int a = 0; a < m->nd; ++a)
+         {
+            if (stamp[(size_t)ed[a]] == e)
+            {
+               return h->fail(LFASM_E_ARG, "mesh %d: element %d holds global dof %d twice", idx,
+                              e, ed[a]);
+            }
+            stamp[(size_t)ed[a]] = e;
+         }
+      }
+   }
    for (int e = 0; e < m->ne; ++e)
    {
       if (m->attr[e] < 1 || m->attr[e] > m->nreg)
@@ -998,11 +1025,12 @@ extern "C" {
 
 int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
 {
-   if (!h || !h->stream) { return LFASM_E_STATE; }
+   if (!h) { return LFASM_E_STATE; }
    if (std::getenv("LITHOFEM_ASM_FORCE_FAIL"))
    {
       return h->fail(LFASM_E_FORCED, "forced failure injected (LITHOFEM_ASM_FORCE_FAIL)");
    }
+   // the input checks run on the host before any device state is needed
    if (!ms || nmesh <= 0) { return h->fail(LFASM_E_ARG, "no mesh"); }
    int rc;
    int64_t ne_tot = 0, n_tot = 0;
@@ -1022,6 +1050,7 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
    {
       return h->fail(LFASM_E_UNSUPPORTED, "batch exceeds int32 element/dof indexing");
    }
+   if (!h->stream) { return h->fail(LFASM_E_STATE, "no device stream (lfasm_create failed)"); }
    LF_HIP(hipSetDevice(h->device));
    release_global(h);
    release_mesh(h);
@@ -1063,7 +1092,11 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
       h->dof_host.assign(m0.dof, m0.dof + ne * nd);
       LF_HIP(upload(h, h->sign, m0.sign, ne * nd));
       if (m0.p >= 2) { LF_HIP(upload(h, h->fo, m0.fo, ne * 4)); }
-      if (m0.eps_elem) { LF_HIP(upload(h, h->eps_elem, (const double2 *)m0.eps_elem, ne)); }
+      if (m0.eps_elem)
+      {
+         LF_HIP(upload(h, h->eps_elem, (const double2 *)m0.eps_elem, ne));
+         h->eps_elem_host.assign((const double2 *)m0.eps_elem, (const double2 *)m0.eps_elem + ne);
+      }
    }
    else
    {
@@ -1106,11 +1139,18 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
       LF_HIP(upload(h, h->sign, sign.data(), sign.size()));
       if (m0.p >= 2) { LF_HIP(upload(h, h->fo, fo.data(), fo.size())); }
       LF_HIP(upload(h, h->pid, pid.data(), pid.size()));
-      if (any_eel) { LF_HIP(upload(h, h->eps_elem, (const double2 *)eel.data(), eel.size() / 2)); }
+      if (any_eel)
+      {
+         LF_HIP(upload(h, h->eps_elem, (const double2 *)eel.data(), eel.size() / 2));
+         h->eps_elem_host.assign((const double2 *)eel.data(),
+                                 (const double2 *)eel.data() + eel.size() / 2);
+      }
    }
    LF_HIP(upload_rule(h, h->rc, m0.rule_cc, m0.nd));
    LF_HIP(upload_rule(h, h->rm, m0.rule_mass, m0.nd));
    LF_HIP(upload(h, h->eps_tab, (const double2 *)eps_tab.data(), eps_tab.size() / 2));
+   h->eps_tab_host.assign((const double2 *)eps_tab.data(),
+                          (const double2 *)eps_tab.data() + eps_tab.size() / 2);
    LF_HIP(upload(h, h->eps_off, eps_off.data(), eps_off.size()));
    LF_HIP(upload(h, h->pml_tab, pml.data(), pml.size()));
    std::memcpy(h->TInv, m0.TInv, sizeof(h->TInv));
@@ -1182,12 +1222,17 @@ int lfasm_set_eps_elem(lfasm_handle *h, const double *eps)
    if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
    LF_HIP(hipSetDevice(h->device));
    drop_graph(h);
+   // the RHS source elements depend on the per-element eps: rebuilt at the
+   // next lfasm_assemble_rhs
+   h->rhs_nsrc = -1;
    if (!eps)
    {
       dfree(h->eps_elem);
+      h->eps_elem_host.clear();
       return LFASM_OK;
    }
    LF_HIP(upload(h, h->eps_elem, (const double2 *)eps, (size_t)h->ne));
+   h->eps_elem_host.assign((const double2 *)eps, (const double2 *)eps + h->ne);
    return LFASM_OK;
 }
 
@@ -1946,6 +1991,8 @@ int lfasm_set_incident(lfasm_handle *h, const lfasm_incident *inc)
    h->inc_nrhs = inc->nrhs;
    h->inc_nslab = inc->nslab;
    h->region_src_host.assign(inc->region_src, inc->region_src + h->nreg);
+   h->region_deps_host.assign((const double2 *)inc->region_deps,
+                              (const double2 *)inc->region_deps + h->nreg);
    dfree(h->rhs_src); dfree(h->rhs_index); dfree(h->rhs_bloc);
    h->rhs_nsrc = -1;
    return LFASM_OK;
@@ -1961,7 +2008,9 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
    const int nrhs = h->inc_nrhs, nd = h->nd;
    if (h->rhs_nsrc < 0)
    {
-      // source elements: those whose region is a source region
+      // source elements: those with a nonzero source strength eps - eps_bg
+      // (k_rhs_local): elements of source regions, and with per-element
+      // eps every element whose eps differs from its slab background
       std::vector<int> attr((size_t)h->ne), pid((size_t)h->ne, 0), eps_off((size_t)h->nmesh);
       LF_HIP(hipMemcpy(attr.data(), h->attr, attr.size() * sizeof(int), hipMemcpyDeviceToHost));
       if (h->pid) { LF_HIP(hipMemcpy(pid.data(), h->pid, pid.size() * sizeof(int), hipMemcpyDeviceToHost)); }
@@ -1970,7 +2019,14 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
       for (int e = 0; e < h->ne; ++e)
       {
          const int reg = eps_off[(size_t)pid[(size_t)e]] + attr[(size_t)e] - 1;
-         if (h->region_src_host[(size_t)reg]) { index[(size_t)e] = (int)src.size(); src.push_back(e); }
+         double2 de = h->region_src_host[(size_t)reg] ? h->region_deps_host[(size_t)reg]
+                                                      : make_double2(0.0, 0.0);
+         if (!h->eps_elem_host.empty())
+         {
+            const double2 ee = h->eps_elem_host[(size_t)e], er = h->eps_tab_host[(size_t)reg];
+            de = make_double2(ee.x - (er.x - de.x), ee.y - (er.y - de.y));
+         }
+         if (de.x != 0.0 || de.y != 0.0) { index[(size_t)e] = (int)src.size(); src.push_back(e); }
       }
       LF_HIP(upload(h, h->rhs_index, index.data(), index.size()));
       LF_HIP(upload(h, h->rhs_src, src.data(), src.size()));
@@ -2015,6 +2071,7 @@ int lfasm_assemble_rhs(lfasm_handle *h, int eliminate)
       R.pml_tab = h->pml_tab;
       R.region_slab = h->region_slab;
       R.region_deps = h->region_deps;
+      R.region_src = h->region_src;
       R.eps_tab = h->eps_tab;
       R.eps_elem = h->eps_elem;
       R.waves = h->waves;

@@ -309,7 +309,8 @@ struct RhsParams {
    RuleDev rule;                  // quadrature rule of the linear form
    const PmlDev *pml_tab;         // k0 per problem
    const int *region_slab;        // per region (concatenated per problem)
-   const double2 *region_deps;    // per region: eps - eps_bg
+   const double2 *region_deps;    // per region: eps - eps_bg (source regions)
+   const int *region_src;         // per region: 1 for scattered-field source regions
    const double2 *eps_tab;        // region eps (concatenated per problem)
    const double2 *eps_elem;       // optional per-element eps
    const SlabWaveDev *waves;      // [pid][rhs][slab]
@@ -329,9 +330,10 @@ __global__ void __launch_bounds__(64) k_rhs_local(RhsParams R)
    const int pid = R.pid ? R.pid[e] : 0;
    const int reg = R.eps_off[pid] + R.attr[e] - 1;
    const int slab = R.region_slab[reg];
-   // source strength eps_e - eps_bg (eps_bg = region eps - region deps);
-   // without per-element eps this is the region's eps - eps_bg itself
-   double2 de = R.region_deps[reg];
+   // source strength eps_e - eps_bg (eps_bg = region eps - region deps,
+   // region deps = 0 outside source regions); without per-element eps this
+   // is the region's eps - eps_bg itself
+   double2 de = R.region_src[reg] ? R.region_deps[reg] : make_double2(0.0, 0.0);
    if (R.eps_elem)
    {
       const double2 ee = R.eps_elem[e], er = R.eps_tab[reg];
@@ -503,7 +505,10 @@ __global__ void __launch_bounds__(64) k_eps_sens(SensParams S)
    const int pid = S.pid ? S.pid[e] : 0;
    const PmlDev pm = S.pml_tab[pid];
    const int reg = S.eps_off[pid] + S.attr[e] - 1;
-   const bool src = S.rhs_col >= 0 && S.region_src && S.region_src[reg];
+   // eps_e enters the element's source strength eps_e - eps_bg linearly in
+   // any region, so db/d eps_e carries the incident field for every
+   // parameter element
+   const bool src = S.rhs_col >= 0;
    const double det = S.detJ[e];
    const double *J = S.J + (size_t)e * 9;
    const double *Ji = S.Ji + (size_t)e * 9;

@@ -1,7 +1,11 @@
 """The reference's ``solve.json`` (src/lithofem/config.py:689-815) as
 problem data for the assembly: region table (attribute = 1-based index into
 ``regions``), PML parameters, background slabs, and per group the Bloch
-vector and the plane-wave fields whose scattered-field RHS the group needs.
+vector and the plane-wave fields whose scattered-field RHS the group needs:
+as descriptions (theta, phi, polarisation, amplitude) that
+lithofem_amd.incident recomputes, and -- when the group carries them, as
+the reference's driver writes it (src/lithofem/driver.py:28-35) -- as the
+reference's own per-slab incident tables, one per source.
 
 Only what the assembly path consumes is read.  Plane waves incident from
 the bottom and point/line/sheet sources are outside this path and raise.
@@ -13,6 +17,7 @@ import json
 from dataclasses import dataclass
 from pathlib import Path
 
+from .incident import tables_from_solve_json
 from .problem import Problem, Region
 
 
@@ -25,8 +30,14 @@ def _cnum(v) -> complex:
 @dataclass
 class GroupCase:
     problem: Problem
-    fields: list          # [(theta, phi, (jones_s, jones_p))] for incident_set
+    fields: list          # [(theta, phi, pol, amplitude)] for incident_set
     source_indices: list
+    tables: list | None = None   # the group's own (nslab, 18) tables, per source
+
+    def incident_fields(self) -> list:
+        """The group's tables when the solve.json holds them, else the
+        descriptions (both are inputs of incident_set)."""
+        return self.tables if self.tables is not None else self.fields
 
 
 def load_solve_json(src) -> list:
@@ -73,10 +84,13 @@ def load_solve_json(src) -> list:
                 raise NotImplementedError("plane waves from the bottom are not on this path")
             amp = _cnum(s.get("amplitude", 1.0))
             pol = s.get("polarization", "s")
-            if s.get("jones"):
-                js, jp = (_cnum(v) for v in s["jones"])
-            else:
-                js, jp = (1.0, 0.0) if pol == "s" else (0.0, 1.0)
-            fields.append((float(s["theta"]), float(s["phi"]), (amp * js, amp * jp)))
-        out.append(GroupCase(pb, fields, list(g["source_indices"])))
+            if pol not in ("s", "p"):
+                # a Jones vector (Ex, Ey) of the transverse incident field
+                pol = tuple(_cnum(v) for v in s["jones"])
+            fields.append((float(s["theta"]), float(s["phi"]), pol, amp))
+        tables = None
+        if g.get("incident"):
+            by_src = tables_from_solve_json(g["incident"], nslab)
+            tables = [by_src[i] for i in g["source_indices"]]
+        out.append(GroupCase(pb, fields, list(g["source_indices"]), tables))
     return out

@@ -285,8 +285,9 @@ static int lower_bound_pos(const int *jb, const int *je, int c)
    return (int)(lo - jb);
 }
 
-/* scatter of one element (solver/asm_data.inc:384-400) */
-static int scatter_one(const orc_asm_data *ad, int e, const C *Ae, double *vre, double *vim)
+/* scatter of one element (solver/asm_data.inc:384-400), rows in [r0, r1) */
+static int scatter_one(const orc_asm_data *ad, int e, const C *Ae, int r0, int r1, double *vre,
+                       double *vim)
 {
    const int nd = ad->nd;
    const int *ed = ad->dof + (size_t)e * nd;
@@ -294,6 +295,7 @@ static int scatter_one(const orc_asm_data *ad, int e, const C *Ae, double *vre, 
    for (int a = 0; a < nd; ++a)
    {
       const int row = ed[a];
+      if (row < r0 || row >= r1) { continue; }
       const int *jb = ad->Jcol + ad->I[row];
       const int *je = ad->Jcol + ad->I[row + 1];
       for (int b = 0; b < nd; ++b)
@@ -325,7 +327,7 @@ int orc_reassemble_all(const orc_asm_data *ad, const orc_problem *pb,
    for (int e = 0; e < ad->ne && rc == 0; ++e)
    {
       reassemble_element_eps(ad, pb, e, mask, 1, NULL, Ae, pc, pv);
-      rc = scatter_one(ad, e, Ae, vre, vim);
+      rc = scatter_one(ad, e, Ae, 0, ad->height, vre, vim);
    }
    free(Ae);
    free(pc);
@@ -341,7 +343,27 @@ int orc_scatter(const orc_asm_data *ad, const double *Ae_all, double *vre, doubl
    memset(vim, 0, nnz * sizeof(double));
    for (int e = 0; e < ad->ne; ++e)
    {
-      if (scatter_one(ad, e, (const C *)(Ae_all + (size_t)e * n2 * 2), vre, vim)) { return -1; }
+      if (scatter_one(ad, e, (const C *)(Ae_all + (size_t)e * n2 * 2), 0, ad->height, vre, vim))
+      {
+         return -1;
+      }
+   }
+   return 0;
+}
+
+int orc_scatter_rows(const orc_asm_data *ad, const double *Ae_all, int r0, int r1, double *vre,
+                     double *vim)
+{
+   const size_t k0 = (size_t)ad->I[r0], k1 = (size_t)ad->I[r1];
+   const size_t n2 = (size_t)ad->nd * ad->nd;
+   memset(vre + k0, 0, (k1 - k0) * sizeof(double));
+   memset(vim + k0, 0, (k1 - k0) * sizeof(double));
+   for (int e = 0; e < ad->ne; ++e)
+   {
+      if (scatter_one(ad, e, (const C *)(Ae_all + (size_t)e * n2 * 2), r0, r1, vre, vim))
+      {
+         return -1;
+      }
    }
    return 0;
 }
@@ -484,18 +506,19 @@ void orc_rhs_element(const orc_asm_data *ad, const orc_problem *pb, const orc_in
    C *b = (C *)b_out;
    for (int k = 0; k < nd; ++k) { b[k] = 0.0; }
    const int reg = ad->attr[e] - 1;
-   if (!inc->region_src[reg]) { return; }
+   /* source strength eps - eps_bg: the region's own (frustum regions,
+    * solver/problem_common.inc:313-317) or, with per-element eps, that of
+    * the element against its slab background eps_bg = eps_reg - deps_reg
+    * (deps_reg = 0 outside source regions) -- so an element of a background
+    * region whose eps differs from the slab's becomes a source as well */
+   C de = inc->region_src[reg] ? ld(inc->region_deps + 2 * reg) : 0.0;
+   if (ad->eps_elem) { de = ld(ad->eps_elem + 2 * (size_t)e) - (ld(pb->eps + 2 * reg) - de); }
+   if (unit_deps) { de = 1.0; }
+   if (de == 0.0) { return; }
    const double *J = ad->J + (size_t)e * 9;
    const double *Ji = ad->Jinv + (size_t)e * 9;
    const double detJ = ad->detJ[e];
    const double *v0 = ad->v0 + (size_t)e * 3;
-   C de = ld(inc->region_deps + 2 * reg);
-   if (ad->eps_elem)
-   {
-      /* per-element generalisation: eps_e - eps_bg, eps_bg = eps_reg - deps_reg */
-      de = ld(ad->eps_elem + 2 * (size_t)e) - (ld(pb->eps + 2 * reg) - de);
-   }
-   if (unit_deps) { de = 1.0; }
    const C fac = pb->k0 * pb->k0 * de;
    const orc_rule *rm = &ad->rule_mass;
    for (int q = 0; q < rm->nq; ++q)
@@ -578,4 +601,248 @@ void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
    free(be);
    free(pc);
    free(pv);
+}
+
+/* ---- componentwise error scale -------------------------------------------
+ * The scale of an entry is the value of its own expression tree evaluated on
+ * the moduli of every factor, with every sum of terms a sum of their moduli
+ * (no cancellation): s >= sum |terms|.  Any evaluation order of the entry in
+ * floating point differs from the exact value by at most about
+ * (number of operations) x 2^-53 x s (the running error bound of the
+ * summation), so |g - r| / s compares two implementations entry by entry
+ * independently of how much the entry cancels. */
+static void element_scale(const orc_asm_data *ad, const orc_problem *pb, int e, int mask,
+                          int apply_te, double *S, double *pc, double *pv)
+{
+   const int nd = ad->nd;
+   const double *J = ad->J + (size_t)e * 9;
+   const double *Ji = ad->Jinv + (size_t)e * 9;
+   const double adet = fabs(ad->detJ[e]);
+   const double *v0 = ad->v0 + (size_t)e * 3;
+   const double aeps = ad->eps_elem ? cabs(ld(ad->eps_elem + 2 * (size_t)e))
+                                    : cabs(ld(pb->eps + 2 * (ad->attr[e] - 1)));
+   const double k2 = pb->k0 * pb->k0;
+   const double aK[9] = {0.0, 0.0, fabs(pb->ky), 0.0, 0.0, fabs(pb->kx),
+                         fabs(pb->ky), fabs(pb->kx), 0.0};
+   for (size_t i = 0; i < (size_t)nd * nd; ++i) { S[i] = 0.0; }
+   if (mask & ORC_CURLCURL)
+   {
+      const orc_rule *rc = &ad->rule_cc;
+      for (int q = 0; q < rc->nq; ++q)
+      {
+         const double *xi = rc->ip + (size_t)q * 3;
+         const double z = v0[2] + J[6] * xi[0] + J[7] * xi[1] + J[8] * xi[2];
+         const C s = s_z(pb, z);
+         const double li[3] = {1.0 / cabs(s), 1.0 / cabs(s), cabs(s)};
+         const double w = fabs(rc->w[q]) * adet;
+         const double *cs = rc->curl + (size_t)q * nd * 3;
+         for (int k = 0; k < nd; ++k)
+            for (int i = 0; i < 3; ++i)
+            {
+               pc[(size_t)k * 3 + i] = (fabs(cs[k * 3 + 0] * J[i * 3 + 0]) +
+                                        fabs(cs[k * 3 + 1] * J[i * 3 + 1]) +
+                                        fabs(cs[k * 3 + 2] * J[i * 3 + 2])) / adet;
+            }
+         for (int k = 0; k < nd; ++k)
+            for (int l = 0; l < nd; ++l)
+            {
+               double t = 0.0;
+               for (int i = 0; i < 3; ++i) { t += pc[(size_t)k * 3 + i] * li[i] * pc[(size_t)l * 3 + i]; }
+               S[(size_t)k * nd + l] += w * t;
+            }
+      }
+   }
+   if (mask & (ORC_MASS | ORC_CROSS0 | ORC_CROSS1 | ORC_KLK))
+   {
+      const orc_rule *rm = &ad->rule_mass;
+      for (int q = 0; q < rm->nq; ++q)
+      {
+         const double *xi = rm->ip + (size_t)q * 3;
+         const double z = v0[2] + J[6] * xi[0] + J[7] * xi[1] + J[8] * xi[2];
+         const C s = s_z(pb, z);
+         const double as = cabs(s);
+         const double lam[3] = {as, as, 1.0 / as};
+         const double li[3] = {1.0 / as, 1.0 / as, as};
+         const double w = fabs(rm->w[q]) * adet;
+         const double *vs = rm->vshape + (size_t)q * nd * 3;
+         const double *cs = rm->curl + (size_t)q * nd * 3;
+         for (int k = 0; k < nd; ++k)
+            for (int i = 0; i < 3; ++i)
+            {
+               pv[(size_t)k * 3 + i] = fabs(vs[k * 3 + 0] * Ji[0 * 3 + i]) +
+                                       fabs(vs[k * 3 + 1] * Ji[1 * 3 + i]) +
+                                       fabs(vs[k * 3 + 2] * Ji[2 * 3 + i]);
+               pc[(size_t)k * 3 + i] = (fabs(cs[k * 3 + 0] * J[i * 3 + 0]) +
+                                        fabs(cs[k * 3 + 1] * J[i * 3 + 1]) +
+                                        fabs(cs[k * 3 + 2] * J[i * 3 + 2])) / adet;
+            }
+         double klk[9];
+         for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+            {
+               klk[r * 3 + c] = 0.0;
+               for (int t = 0; t < 3; ++t) { klk[r * 3 + c] += aK[r * 3 + t] * li[t] * aK[t * 3 + c]; }
+            }
+         for (int k = 0; k < nd; ++k)
+            for (int l = 0; l < nd; ++l)
+            {
+               double t = 0.0;
+               if (mask & ORC_MASS)
+               {
+                  for (int i = 0; i < 3; ++i)
+                  {
+                     t += pv[(size_t)k * 3 + i] * (k2 * aeps * lam[i]) * pv[(size_t)l * 3 + i];
+                  }
+               }
+               if (ad->has_kpar)
+               {
+                  for (int r = 0; r < 3; ++r)
+                     for (int c = 0; c < 3; ++c)
+                     {
+                        if (mask & ORC_CROSS0)
+                        {
+                           t += pv[(size_t)k * 3 + r] * aK[r * 3 + c] * li[c] * pc[(size_t)l * 3 + c];
+                        }
+                        if (mask & ORC_CROSS1)
+                        {
+                           t += pc[(size_t)k * 3 + r] * li[r] * aK[r * 3 + c] * pv[(size_t)l * 3 + c];
+                        }
+                        if (mask & ORC_KLK)
+                        {
+                           t += pv[(size_t)k * 3 + r] * klk[r * 3 + c] * pv[(size_t)l * 3 + c];
+                        }
+                     }
+               }
+               S[(size_t)k * nd + l] += w * t;
+            }
+      }
+   }
+   if (apply_te && ad->p >= 2)
+   {
+      const int p = ad->p, foff = 6 * p, ntd = p * (p - 1);
+      const int *fo = ad->fo + (size_t)e * 4;
+      for (int axis = 0; axis < 2; ++axis)
+         for (int f = 0; f < 4; ++f)
+         {
+            const double *Ti = orc_TInv + 4 * fo[f];
+            for (int i = 0; i < ntd / 2; ++i)
+            {
+               const int d0 = foff + f * ntd + 2 * i;
+               for (int r = 0; r < nd; ++r)
+               {
+                  double *a = axis == 0 ? &S[(size_t)r * nd + d0] : &S[(size_t)d0 * nd + r];
+                  double *b = axis == 0 ? &S[(size_t)r * nd + d0 + 1] : &S[(size_t)(d0 + 1) * nd + r];
+                  const double na = fabs(Ti[0]) * *a + fabs(Ti[1]) * *b;
+                  const double nb = fabs(Ti[2]) * *a + fabs(Ti[3]) * *b;
+                  *a = na;
+                  *b = nb;
+               }
+            }
+         }
+   }
+}
+
+void orc_scale_range(const orc_asm_data *ad, const orc_problem *pb, int e0, int e1, int mask,
+                     int apply_te, double *S_out)
+{
+   const int nd = ad->nd;
+   double *pc = (double *)malloc(sizeof(double) * 3 * nd);
+   double *pv = (double *)malloc(sizeof(double) * 3 * nd);
+   for (int e = e0; e < e1; ++e)
+   {
+      element_scale(ad, pb, e, mask, apply_te, S_out + (size_t)(e - e0) * nd * nd, pc, pv);
+   }
+   free(pc);
+   free(pv);
+}
+
+int orc_scale_scatter_rows(const orc_asm_data *ad, const double *S_all, int r0, int r1,
+                           double *s)
+{
+   const int nd = ad->nd;
+   const size_t k0 = (size_t)ad->I[r0], k1 = (size_t)ad->I[r1];
+   memset(s + k0, 0, (k1 - k0) * sizeof(double));
+   for (int e = 0; e < ad->ne; ++e)
+   {
+      const int *ed = ad->dof + (size_t)e * nd;
+      const double *Se = S_all + (size_t)e * nd * nd;
+      for (int a = 0; a < nd; ++a)
+      {
+         const int row = ed[a];
+         if (row < r0 || row >= r1) { continue; }
+         const int *jb = ad->Jcol + ad->I[row];
+         const int *je = ad->Jcol + ad->I[row + 1];
+         for (int b = 0; b < nd; ++b)
+         {
+            const int off = lower_bound_pos(jb, je, ed[b]);
+            if (jb + off == je || jb[off] != ed[b]) { return -1; }
+            s[(size_t)ad->I[row] + off] += Se[(size_t)a * nd + b];
+         }
+      }
+   }
+   return 0;
+}
+
+void orc_rhs_scale_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
+                       int rhs, double *s)
+{
+   const int nd = ad->nd;
+   for (int i = 0; i < ad->height; ++i) { s[i] = 0.0; }
+   double *be = (double *)malloc(sizeof(double) * nd);
+   for (int e = 0; e < ad->ne; ++e)
+   {
+      const int reg = ad->attr[e] - 1;
+      C de = inc->region_src[reg] ? ld(inc->region_deps + 2 * reg) : 0.0;
+      if (ad->eps_elem) { de = ld(ad->eps_elem + 2 * (size_t)e) - (ld(pb->eps + 2 * reg) - de); }
+      if (de == 0.0) { continue; }
+      const double *J = ad->J + (size_t)e * 9;
+      const double *Ji = ad->Jinv + (size_t)e * 9;
+      const double adet = fabs(ad->detJ[e]);
+      const double *v0 = ad->v0 + (size_t)e * 3;
+      const double fac = pb->k0 * pb->k0 * cabs(de);
+      const orc_rule *rm = &ad->rule_mass;
+      for (int k = 0; k < nd; ++k) { be[k] = 0.0; }
+      for (int q = 0; q < rm->nq; ++q)
+      {
+         const double *xi = rm->ip + (size_t)q * 3;
+         const double z = v0[2] + J[6] * xi[0] + J[7] * xi[1] + J[8] * xi[2];
+         const double *w = inc->waves + 18 * ((size_t)rhs * inc->nslab + inc->region_slab[reg]);
+         const double up = cabs(cexp(IU * ld(w + 2) * (z - w[0])));
+         const double dn = cabs(cexp(IU * ld(w + 4) * (z - w[1])));
+         double ae[3];
+         for (int i = 0; i < 3; ++i) { ae[i] = cabs(ld(w + 6 + 2 * i)) * up + cabs(ld(w + 12 + 2 * i)) * dn; }
+         const double wq = fabs(rm->w[q]) * adet;
+         const double *vs = rm->vshape + (size_t)q * nd * 3;
+         for (int k = 0; k < nd; ++k)
+         {
+            double t = 0.0;
+            for (int i = 0; i < 3; ++i)
+            {
+               const double v = fabs(vs[k * 3 + 0] * Ji[0 * 3 + i]) + fabs(vs[k * 3 + 1] * Ji[1 * 3 + i]) +
+                                fabs(vs[k * 3 + 2] * Ji[2 * 3 + i]);
+               t += v * fac * ae[i];
+            }
+            be[k] += wq * t;
+         }
+      }
+      if (ad->p >= 2)
+      {
+         const int p = ad->p, foff = 6 * p, ntd = p * (p - 1);
+         const int *fo = ad->fo + (size_t)e * 4;
+         for (int f = 0; f < 4; ++f)
+         {
+            const double *Ti = orc_TInv + 4 * fo[f];
+            for (int i = 0; i < ntd / 2; ++i)
+            {
+               const int d0 = foff + f * ntd + 2 * i;
+               const double a = be[d0], c = be[d0 + 1];
+               be[d0] = fabs(Ti[0]) * a + fabs(Ti[1]) * c;
+               be[d0 + 1] = fabs(Ti[2]) * a + fabs(Ti[3]) * c;
+            }
+         }
+      }
+      const int *ed = ad->dof + (size_t)e * nd;
+      for (int a = 0; a < nd; ++a) { s[ed[a]] += be[a]; }
+   }
+   free(be);
 }

@@ -82,6 +82,13 @@ int orc_reassemble_all(const orc_asm_data *ad, const orc_problem *pb,
 int orc_scatter(const orc_asm_data *ad, const double *Ae_all,
                 double *vre, double *vim);
 
+/* orc_scatter restricted to the CSR rows [r0, r1): zeroes and fills only
+ * their values, each entry accumulated in the same ascending element order
+ * (disjoint row ranges may run on separate host threads; their union is
+ * bit-identical to orc_scatter).  (pinned: equals orc_scatter) */
+int orc_scatter_rows(const orc_asm_data *ad, const double *Ae_all, int r0, int r1,
+                     double *vre, double *vim);
+
 /* solver/asm_data.inc:178-210: I (height+1) filled, *Jcol_out malloc'ed.
  * Returns nnz, or -1.  (pinned: brute force on small meshes) */
 int64_t orc_build_csr_symbolic(int ne, int nd, int height, const int *dof,
@@ -110,8 +117,11 @@ typedef struct {
 
 /* RHS element vector of incident field `rhs` (ScatterSource,
  * solver/problem_common.inc:238-266) on the mass rule with the dual face
- * transform (nd interleaved complex); unit_deps = 1 replaces the region's
- * eps - eps_bg by 1 (the derivative with respect to eps).  (pinned) */
+ * transform (nd interleaved complex).  Source strength eps - eps_bg: the
+ * region's deps in source regions, 0 elsewhere; with per-element eps,
+ * eps_e - (eps_reg - deps_reg) in every region (an element whose eps
+ * differs from its slab background is a source).  unit_deps = 1 replaces
+ * it by 1 in any region (the derivative with respect to eps_e).  (pinned) */
 void orc_rhs_element(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
                      int e, int rhs, int unit_deps, double *b);
 
@@ -123,11 +133,26 @@ void orc_rhs_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incide
  * central finite differences of the assembled matrix and RHS): for each
  * element e, g[e] = lam^T (dA/d eps_e u - db/d eps_e), complex, where
  * dA/d eps_e is the mass term with eps = 1 and db/d eps_e the RHS of
- * incident field rhs_col at unit eps - eps_bg (source regions only;
+ * incident field rhs_col at unit eps - eps_bg (every element: eps_e enters
+ * its source strength eps_e - eps_bg linearly in any region;
  * inc == NULL or rhs_col < 0: matrix term only). */
 void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
                          const orc_incident *inc, int rhs_col,
                          const double *lam, const double *u, double *g);
+
+/* Componentwise error scales (test metric, no reference counterpart): each
+ * entry's expression evaluated on the moduli of its factors, with every sum
+ * a sum of moduli, so that s >= sum |terms| and any two evaluation orders of
+ * the entry differ by at most about (operations) x 2^-53 x s.  Element
+ * scales (real, nd*nd per element, same mask / transform meaning as
+ * orc_reassemble_range); their scatter over CSR rows [r0, r1) (unit signs);
+ * the RHS scale of incident field `rhs` (n reals). */
+void orc_scale_range(const orc_asm_data *ad, const orc_problem *pb, int e0, int e1, int mask,
+                     int apply_te, double *S_out);
+int orc_scale_scatter_rows(const orc_asm_data *ad, const double *S_all, int r0, int r1,
+                           double *s);
+void orc_rhs_scale_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
+                       int rhs, double *s);
 
 #ifdef __cplusplus
 }

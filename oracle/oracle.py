@@ -57,15 +57,16 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
-_lib = None
+_lib = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB.exists():
+def lib(fast: bool = False):
+    """The parity build (liborc.so) or the -O3 FMA build (liborc_fast.so)."""
+    if fast not in _lib:
+        path = HERE / ("liborc_fast.so" if fast else "liborc.so")
+        if not path.exists():
             build()
-        L = ct.CDLL(str(LIB))
+        L = ct.CDLL(str(path))
         L.orc_s_z.argtypes = [ct.c_void_p, ct.c_double, ct.c_void_p]
         L.orc_reassemble_element.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
                                              ct.c_int, ct.c_void_p]
@@ -75,6 +76,16 @@ def lib():
         L.orc_reassemble_all.restype = ct.c_int
         L.orc_scatter.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.orc_scatter.restype = ct.c_int
+        L.orc_scatter_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                       ct.c_void_p, ct.c_void_p]
+        L.orc_scatter_rows.restype = ct.c_int
+        L.orc_scale_range.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int,
+                                      ct.c_int, ct.c_void_p]
+        L.orc_scale_scatter_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                             ct.c_void_p]
+        L.orc_scale_scatter_rows.restype = ct.c_int
+        L.orc_rhs_scale_all.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                        ct.c_void_p]
         L.orc_build_csr_symbolic.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_void_p,
                                              ct.c_void_p, ct.POINTER(ct.c_void_p)]
         L.orc_build_csr_symbolic.restype = ct.c_int64
@@ -88,16 +99,17 @@ def lib():
         L.orc_rhs_element.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
                                       ct.c_int, ct.c_void_p]
         L.orc_rhs_all.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
-        _lib = L
-    return _lib
+        _lib[fast] = L
+    return _lib[fast]
 
 
 class Oracle:
     """Holds the ctypes views of one (AsmData, Problem) pair."""
 
-    def __init__(self, ad, pb, I=None, Jcol=None, eps_elem=None):
+    def __init__(self, ad, pb, I=None, Jcol=None, eps_elem=None, fast=False):
         self.ad, self.pb = ad, pb
         self._keep = []
+        self.L = lib(fast)
 
         def keep(a, dt):
             if a is None:
@@ -132,20 +144,20 @@ class Oracle:
 
     def s_z(self, z: float) -> complex:
         out = np.zeros(2)
-        lib().orc_s_z(ct.byref(self.prob), z, _ptr(out))
+        self.L.orc_s_z(ct.byref(self.prob), z, _ptr(out))
         return complex(out[0], out[1])
 
     def element(self, e: int, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
         nd = self.ad.nd
         out = np.zeros((nd, nd), dtype=np.complex128)
-        lib().orc_reassemble_element(ct.byref(self.data), ct.byref(self.prob), e, mask,
+        self.L.orc_reassemble_element(ct.byref(self.data), ct.byref(self.prob), e, mask,
                                      1 if apply_te else 0, _ptr(out))
         return out
 
     def elements(self, e0: int, e1: int, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
         nd = self.ad.nd
         out = np.zeros((e1 - e0, nd, nd), dtype=np.complex128)
-        lib().orc_reassemble_range(ct.byref(self.data), ct.byref(self.prob), e0, e1, mask,
+        self.L.orc_reassemble_range(ct.byref(self.data), ct.byref(self.prob), e0, e1, mask,
                                    1 if apply_te else 0, _ptr(out))
         return out
 
@@ -153,7 +165,7 @@ class Oracle:
         nnz = int(self.I[-1])
         vre = np.zeros(nnz)
         vim = np.zeros(nnz)
-        rc = lib().orc_reassemble_all(ct.byref(self.data), ct.byref(self.prob),
+        rc = self.L.orc_reassemble_all(ct.byref(self.data), ct.byref(self.prob),
                                       _ptr(vre), _ptr(vim))
         if rc != 0:
             raise RuntimeError("CSR position missing")
@@ -164,10 +176,99 @@ class Oracle:
         vre = np.zeros(nnz)
         vim = np.zeros(nnz)
         Ae_all = np.ascontiguousarray(Ae_all, dtype=np.complex128)
-        rc = lib().orc_scatter(ct.byref(self.data), _ptr(Ae_all), _ptr(vre), _ptr(vim))
+        rc = self.L.orc_scatter(ct.byref(self.data), _ptr(Ae_all), _ptr(vre), _ptr(vim))
         if rc != 0:
             raise RuntimeError("CSR position missing")
         return vre, vim
+
+    def elements_threaded(self, nthreads: int, mask: int = ALL, apply_te: bool = True):
+        """All element matrices, contiguous element blocks on host threads
+        (each thread writes its own block of one output array)."""
+        from concurrent.futures import ThreadPoolExecutor
+        ne, nd = self.ad.ne, self.ad.nd
+        out = np.zeros((ne, nd, nd), dtype=np.complex128)
+        cuts = [(ne * i) // nthreads for i in range(nthreads + 1)]
+        stride = nd * nd * 16
+
+        def run(i):
+            e0, e1 = cuts[i], cuts[i + 1]
+            if e1 > e0:
+                self.L.orc_reassemble_range(ct.byref(self.data), ct.byref(self.prob), e0, e1,
+                                           mask, 1 if apply_te else 0,
+                                           out.ctypes.data + e0 * stride)
+
+        with ThreadPoolExecutor(nthreads) as ex:
+            list(ex.map(run, range(nthreads)))
+        return out
+
+    def scatter_threaded(self, Ae_all: np.ndarray, nthreads: int):
+        """orc_scatter over disjoint CSR row ranges on host threads
+        (bit-identical to scatter())."""
+        from concurrent.futures import ThreadPoolExecutor
+        nnz = int(self.I[-1])
+        n = self.ad.height
+        vre = np.empty(nnz)
+        vim = np.empty(nnz)
+        Ae_all = np.ascontiguousarray(Ae_all, dtype=np.complex128)
+        cuts = [(n * i) // nthreads for i in range(nthreads + 1)]
+        rcs = []
+
+        def run(i):
+            rcs.append(self.L.orc_scatter_rows(ct.byref(self.data), _ptr(Ae_all), cuts[i],
+                                              cuts[i + 1], _ptr(vre), _ptr(vim)))
+
+        with ThreadPoolExecutor(nthreads) as ex:
+            list(ex.map(run, range(nthreads)))
+        if any(rcs):
+            raise RuntimeError("CSR position missing")
+        return vre, vim
+
+    # ---- componentwise error scales (see orc_scale_range)
+    def scale_elements(self, elems, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
+        """Scales of the element matrices of the given elements, (n, nd, nd)."""
+        nd = self.ad.nd
+        elems = np.atleast_1d(np.asarray(elems, dtype=np.int64))
+        out = np.zeros((len(elems), nd, nd))
+        for i, e in enumerate(elems):
+            self.L.orc_scale_range(ct.byref(self.data), ct.byref(self.prob), int(e), int(e) + 1,
+                                  mask, 1 if apply_te else 0, out.ctypes.data + i * nd * nd * 8)
+        return out
+
+    def scale_csr(self, nthreads: int = 1, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
+        """Scales of the CSR values (nnz): element scales summed over the
+        contributions of each nonzero; element blocks and row ranges on
+        host threads."""
+        from concurrent.futures import ThreadPoolExecutor
+        ne, nd, n = self.ad.ne, self.ad.nd, self.ad.height
+        S = np.zeros((ne, nd, nd))
+        cuts = [(ne * i) // nthreads for i in range(nthreads + 1)]
+
+        def run_el(i):
+            if cuts[i + 1] > cuts[i]:
+                self.L.orc_scale_range(ct.byref(self.data), ct.byref(self.prob), cuts[i],
+                                      cuts[i + 1], mask, 1 if apply_te else 0,
+                                      S.ctypes.data + cuts[i] * nd * nd * 8)
+
+        s = np.empty(int(self.I[-1]))
+        rcut = [(n * i) // nthreads for i in range(nthreads + 1)]
+        rcs = []
+
+        def run_rows(i):
+            rcs.append(self.L.orc_scale_scatter_rows(ct.byref(self.data), _ptr(S), rcut[i],
+                                                    rcut[i + 1], _ptr(s)))
+
+        with ThreadPoolExecutor(nthreads) as ex:
+            list(ex.map(run_el, range(nthreads)))
+            list(ex.map(run_rows, range(nthreads)))
+        if any(rcs):
+            raise RuntimeError("CSR position missing")
+        return s
+
+    def rhs_scale(self, r: int) -> np.ndarray:
+        s = np.zeros(self.ad.height)
+        self.L.orc_rhs_scale_all(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), r,
+                                _ptr(s))
+        return s
 
     def set_incident(self, inc):
         """inc: problem-local incident set (one problem): nrhs, nslab, waves
@@ -182,12 +283,12 @@ class Oracle:
 
     def rhs(self, r: int) -> np.ndarray:
         b = np.zeros(self.ad.height, dtype=np.complex128)
-        lib().orc_rhs_all(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), r, _ptr(b))
+        self.L.orc_rhs_all(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), r, _ptr(b))
         return b
 
     def rhs_element(self, e: int, r: int, unit: bool = False) -> np.ndarray:
         b = np.zeros(self.ad.nd, dtype=np.complex128)
-        lib().orc_rhs_element(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), e, r,
+        self.L.orc_rhs_element(ct.byref(self.data), ct.byref(self.prob), ct.byref(self.inc), e, r,
                               1 if unit else 0, _ptr(b))
         return b
 
@@ -196,7 +297,7 @@ class Oracle:
         u = np.ascontiguousarray(u, dtype=np.complex128)
         g = np.zeros(self.ad.ne, dtype=np.complex128)
         inc = ct.byref(self.inc) if rhs_col >= 0 else None
-        lib().orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob), inc, rhs_col,
+        self.L.orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob), inc, rhs_col,
                                   _ptr(lam), _ptr(u), _ptr(g))
         return g
 
@@ -223,6 +324,14 @@ def eliminate(I, Jcol, is_ess, vre, vim):
     ess = np.ascontiguousarray(is_ess, dtype=np.uint8)
     lib().orc_eliminate(len(I) - 1, _ptr(I), _ptr(Jcol), _ptr(ess), _ptr(vre), _ptr(vim))
     return vre, vim
+
+
+def eliminate_scale(I, Jcol, is_ess, s):
+    """Scale of the eliminated system: essential rows and columns are set,
+    not computed (0 off the diagonal, 1 on it)."""
+    z = np.zeros_like(s)
+    out_re, _ = eliminate(I, Jcol, is_ess, s, z)
+    return out_re
 
 
 def residual(I, Jcol, vre, vim, X2n, B2n) -> float:

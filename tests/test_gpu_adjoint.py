@@ -28,7 +28,7 @@ def _setup(p=2):
     pb = c.problem
     ad = build_asm_data(c.mesh, p)
     I, J = csr_symbolic(ad.dof, ad.height)
-    inc = incident_set([pb], [[(6.0, 0.0, (1.0, 0.0))]])
+    inc = incident_set([pb], [[(6.0, 0.0, "s")]])
     params = np.nonzero((ad.attr == 2) | (ad.attr == 3))[0].astype(np.int32)
     model = EpsForwardModel(ad, pb, I, J, params, incident=inc, rhs_col=0)
     rng = np.random.default_rng(3)

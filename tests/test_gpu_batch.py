@@ -12,10 +12,10 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from cases import maxnorm_err
+from cases import assert_parity, maxnorm_err
 from lithofem_amd import gpu
 from lithofem_amd.fespace import build_asm_data, csr_symbolic
-from lithofem_amd.incident import incident_set
+from lithofem_amd.incident import incident_set, sp_coefficients
 from lithofem_amd.problem import contact_hole, line_grating
 from oracle import oracle as O
 
@@ -76,7 +76,8 @@ def test_multi_mesh_batch_equals_single(p):
         is_ess = np.zeros(ad.height, np.uint8)
         is_ess[ad.ess] = 1
         rre, rim = O.eliminate(Ii, Ji, is_ess, rre, rim)
-        assert maxnorm_err(sre + 1j * sim, rre + 1j * rim) < 1e-13
+        sc = O.eliminate_scale(Ii, Ji, is_ess, orc.scale_csr(4))
+        assert_parity(sre + 1j * sim, rre + 1j * rim, 1e-13, f"mesh {c.problem.kx:.3f}", sc)
         pos += nn
 
 
@@ -128,7 +129,7 @@ def test_batched_rhs_matches_oracle(name, p):
     ad = build_asm_data(c.mesh, p)
     I, J = csr_symbolic(ad.dof, ad.height)
     phi = 0.0 if name == "grating" else 30.0
-    fields = [(6.0, phi, (1.0, 0.0)), (6.0, phi, (0.0, 1.0)),
+    fields = [(6.0, phi, "s"), (6.0, phi, "p"),
               (6.0, phi, (1 / np.sqrt(2), 1j / np.sqrt(2)))]
     inc = incident_set([pb], [fields])
     g = gpu.AsmGpuGlobal(0)
@@ -153,10 +154,12 @@ def test_batched_rhs_matches_oracle(name, p):
     for r in range(len(fields)):
         ref = orc.rhs(r)
         ref[ad.ess] = 0.0
-        err = maxnorm_err(b[r], ref)
-        assert err < 1e-13, (r, err)
-    # Jones (1, i)/sqrt(2) is the linear combination of s and p
-    assert maxnorm_err(b[2], (b[0] + 1j * b[1]) / np.sqrt(2)) < 1e-13
+        sc = orc.rhs_scale(r)
+        sc[ad.ess] = 0.0
+        assert_parity(b[r], ref, 1e-13, f"rhs {r}", sc)
+    # Jones (1, i)/sqrt(2) is the linear combination c_s s + c_p p
+    cs, cp = sp_coefficients(6.0, phi, (1 / np.sqrt(2), 1j / np.sqrt(2)))
+    assert maxnorm_err(b[2], cs * b[0] + cp * b[1]) < 1e-13
 
 
 @pytest.mark.parametrize("p", [2, 3])
@@ -165,7 +168,7 @@ def test_eps_sensitivity_matches_oracle(p):
     pb = c.problem
     ad = build_asm_data(c.mesh, p)
     I, J = csr_symbolic(ad.dof, ad.height)
-    inc = incident_set([pb], [[(6.0, 0.0, (0.0, 1.0))]])
+    inc = incident_set([pb], [[(6.0, 0.0, "p")]])
     rng = np.random.default_rng(9)
     lam = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
     u = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)

@@ -16,6 +16,7 @@
 from __future__ import annotations
 
 import dataclasses
+import json
 import math
 import re
 import subprocess
@@ -72,6 +73,31 @@ def test_library_csr_symbolic_equals_numpy(lib, p):
     assert len(keys) == len(J1)
     got = np.repeat(np.arange(ad.height), np.diff(I1)).astype(np.int64) * ad.height + J1
     assert np.array_equal(np.sort(got), keys)
+
+
+def test_library_rejects_repeated_dof_in_an_element(lib):
+    """lfasm_init validates the mesh on the host before it needs the device:
+    an element holding one global dof at two local positions is refused
+    (the row scatter would lose one of its contributions), a valid mesh
+    passes validation and then reports the missing device."""
+    import ctypes as ct
+    c = contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=0.0)
+    ad = build_asm_data(c.mesh, 2)
+    h = ct.c_void_p()
+    lib.lfasm_create(0, ct.byref(h))          # no device here: the handle holds the error
+    try:
+        keep = []
+        m = gpu.AsmGpuGlobal._mesh_struct(None, ad, c.problem, None, keep)
+        assert lib.lfasm_init(h, ct.byref(m)) == gpu.E_STATE
+        bad = ad.dof.copy()
+        bad[5, 7] = bad[5, 2]
+        ad_bad = dataclasses.replace(ad, dof=bad)
+        m = gpu.AsmGpuGlobal._mesh_struct(None, ad_bad, c.problem, None, keep)
+        assert lib.lfasm_init(h, ct.byref(m)) == gpu.E_ARG
+        msg = lib.lfasm_last_error(h).decode()
+        assert "element 5 holds global dof" in msg, msg
+    finally:
+        lib.lfasm_destroy(h)
 
 
 def test_library_pml_smax(lib):
@@ -167,7 +193,7 @@ def test_nedelec_space_is_tangentially_conforming(name, p):
 
 
 @pytest.mark.parametrize("phi", [0.0, 30.0])
-@pytest.mark.parametrize("jones", [(1.0, 0.0), (0.0, 1.0)])
+@pytest.mark.parametrize("jones", ["s", "p"])
 def test_incident_field_interface_conditions(phi, jones):
     c = contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=9.0, phi=phi)
     pb = c.problem
@@ -211,6 +237,71 @@ def test_incident_field_interface_conditions(phi, jones):
     # unit incident amplitude from the top (|E_inc| = 1 for s, |H_inc| ~ n for p)
     E_in = W[-1][12:18:2] + 1j * W[-1][13:18:2]
     assert abs(np.linalg.norm(E_in) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_sampling_matrix_reproduces_fields_in_the_space(p):
+    """S @ g equals the field at the points for a field of the discrete
+    space: on the (non-periodic) warped unit cube every polynomial of
+    degree p - 1 is in it (a random complex linear field for p >= 2); on
+    the periodic grating cell a constant field, with points outside the
+    cell wrapped back by `periodic`."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import mms
+    from lithofem_amd.adjoint import sampling_matrix
+    from lithofem_amd.fespace import interpolate
+    rng = np.random.default_rng(50 + p)
+    a = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    B = (rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))) if p >= 2 else 0 * np.eye(3)
+
+    def F(X):
+        return a[None, :] + X @ np.asarray(B).T
+
+    ad = build_asm_data(mms.cube_mesh(3), p)
+    pts = rng.uniform(0.02, 0.98, (200, 3))
+    S = sampling_matrix(ad, pts)
+    got = (S @ interpolate(ad, F)).reshape(-1, 3)
+    assert np.abs(got - F(pts)).max() < 1e-12 * np.abs(F(pts)).max()
+    c = line_grating(nx=3, ny=3, nz_scale=0.1, theta=6.0)
+    adg = build_asm_data(c.mesh, p)
+    pts = np.stack([rng.uniform(-64, 128, 100), rng.uniform(-8, 16, 100),
+                    rng.uniform(-10, 130, 100)], 1)
+    S = sampling_matrix(adg, pts, periodic=(64.0, 8.0))
+    got = (S @ interpolate(adg, lambda X: np.tile(a, (len(X), 1)))).reshape(-1, 3)
+    assert np.abs(got - a[None, :]).max() < 1e-12 * np.abs(a).max()
+
+
+def test_order_efficiencies_single_plane_wave():
+    """A plane wave of order (m, n) sampled on the cell grid has one Fourier
+    coefficient: its efficiency is |a|^2 kz_mn / kz_inc, every other order
+    is 0; a sum of two orders gives both (the per-order flux of the
+    reference's orders_from_plane / one_way_flux_z,
+    src/lithofem/orders.py:36-92)."""
+    import torch
+    from lithofem_amd.adjoint import order_efficiencies, plane_points
+    k0 = 2 * math.pi / 13.5
+    lx, ly, nx, ny = 64.0, 8.0, 16, 4
+    kpar = (k0 * math.sin(math.radians(6.0)), 0.0)
+    pts = plane_points(lx, ly, 130.0, nx, ny, nudge=(0.0, 0.0, 0.0))
+    x, y = pts[:, 0].reshape(nx, ny), pts[:, 1].reshape(nx, ny)
+    a1 = np.array([0.2 + 0.1j, -0.05j, 0.3])
+    a2 = np.array([0.1, 0.4 - 0.2j, -0.1j])
+
+    def wave(m, n, amp):
+        ph = np.exp(2j * math.pi * (m * x / lx + n * y / ly))
+        return ph[..., None] * amp[None, None, :]
+
+    E = torch.as_tensor(wave(1, 0, a1) + wave(-1, 0, a2))
+    eff = order_efficiencies(E, kpar, k0, lx, ly, [(1, 0), (-1, 0), (0, 0), (2, 0)])
+
+    def kz(m):
+        kx = kpar[0] + 2 * math.pi * m / lx
+        return math.sqrt(k0 * k0 - kx * kx)
+
+    want = [np.sum(np.abs(a1) ** 2) * kz(1) / kz(0), np.sum(np.abs(a2) ** 2) * kz(-1) / kz(0),
+            0.0, 0.0]
+    for g, w in zip(eff, want):
+        assert abs(float(g) - w) < 1e-14
 
 
 def test_bench_refuses_more_than_one_gpu():
@@ -268,7 +359,7 @@ def _fields(W, j, z):
     return evaluate(W, j, np.atleast_1d(z))[0]
 
 
-@pytest.mark.parametrize("pol", [(1.0, 0.0), (0.0, 1.0)])
+@pytest.mark.parametrize("pol", ["s", "p"])
 def test_incident_uniform_background_is_bare_wave(pol):
     """Vacuum everywhere: no reflected wave, |E| = 1, phase advance
     exp(-i qz dz) (the reference's tests/test_incident.py:78-97)."""
@@ -290,17 +381,99 @@ def test_incident_polarisation_conventions():
     """s at normal incidence: E = +y, amplitude 1 at z_max; oblique p: E at
     z_max = e_p = k_hat x e_s (tests/test_incident.py:117-140)."""
     pb = _stack([(0.0, 60.0, 1.0), (60.0, 120.0, 1.0)])
-    e = _fields(slab_waves(pb, 0.0, 0.0, (1.0, 0.0)), 1, 120.0)
+    e = _fields(slab_waves(pb, 0.0, 0.0, "s"), 1, 120.0)
     assert np.allclose(e, [0.0, 1.0, 0.0], atol=1e-13)
     th, ph = math.radians(30.0), math.radians(40.0)
     k_hat = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), -math.cos(th)])
     e_s = np.cross([0.0, 0.0, 1.0], k_hat)
     e_s /= np.linalg.norm(e_s)
     e_p = np.cross(k_hat, e_s)
-    e = _fields(slab_waves(pb, 30.0, 40.0, (0.0, 1.0)), 1, 120.0)
+    e = _fields(slab_waves(pb, 30.0, 40.0, "p"), 1, 120.0)
     assert np.linalg.norm(e - e_p) < 1e-12
-    e = _fields(slab_waves(pb, 30.0, 40.0, (1.0, 0.0)), 1, 120.0)
+    e = _fields(slab_waves(pb, 30.0, 40.0, "s"), 1, 120.0)
     assert np.linalg.norm(e - e_s) < 1e-12
+
+
+def test_jones_vector_decomposition_closed_form():
+    """A Jones vector is the transverse (Ex, Ey) of the incident field,
+    decomposed onto (e_s, e_p) (src/lithofem/incident.py:220-225).  At
+    phi = 0, e_s = y and e_p = (cos t, 0, sin t), so c_s = Ey and
+    c_p = Ex / cos t: Jones (1, 0) is pure p, (0, 1) pure s, and
+    (1, i)/sqrt 2 gives c_s = i/sqrt 2, c_p = 1/(sqrt 2 cos t)."""
+    from lithofem_amd.incident import sp_coefficients
+    r2 = 1.0 / math.sqrt(2.0)
+    ct6 = math.cos(math.radians(6.0))
+    cs, cp = sp_coefficients(6.0, 0.0, (r2, 1j * r2))
+    assert abs(cs - 1j * r2) < 1e-15 and abs(cp - r2 / ct6) < 1e-15
+    cs, cp = sp_coefficients(6.0, 0.0, (1.0, 0.0))
+    assert abs(cs) < 1e-16 and abs(cp - 1.0 / ct6) < 1e-15
+    cs, cp = sp_coefficients(6.0, 0.0, (0.0, 1.0))
+    assert abs(cs - 1.0) < 1e-16 and abs(cp) < 1e-16
+    assert sp_coefficients(6.0, 0.0, "s") == (1.0, 0.0)
+    assert sp_coefficients(6.0, 0.0, "p") == (0.0, 1.0)
+
+
+@pytest.mark.parametrize("theta,phi", [(6.0, 0.0), (14.0, 35.0), (0.0, 30.0)])
+def test_jones_field_at_the_top(theta, phi):
+    """In a uniform vacuum stack the field at z_max is the incident wave
+    alone: its transverse part is amplitude x (Ex, Ey), and it is
+    transverse to k_hat; at normal incidence s is along +y whatever phi
+    (the reference's basis, src/lithofem/incident.py:82-97)."""
+    from lithofem_amd.incident import sp_basis
+    pb = _stack([(0.0, 60.0, 1.0), (60.0, 120.0, 1.0)])
+    jones = (0.3 - 0.2j, 0.8 + 0.1j)
+    amp = 0.7 * np.exp(0.4j)
+    e = _fields(slab_waves(pb, theta, phi, jones, amp), 1, 120.0)
+    assert np.abs(e[:2] - amp * np.array(jones)).max() < 1e-13
+    _, _, k_hat = sp_basis(theta, phi)
+    assert abs(e @ k_hat) < 1e-13
+    if theta == 0.0:
+        assert np.allclose(_fields(slab_waves(pb, 0.0, phi, "s"), 1, 120.0), [0, 1, 0],
+                           atol=1e-14)
+        assert np.allclose(_fields(slab_waves(pb, 0.0, phi, "p"), 1, 120.0), [1, 0, 0],
+                           atol=1e-14)
+
+
+def test_incident_tables_reference_layout():
+    """groups[].incident of the reference's solve.json: one part per pure
+    s / p component of a source, summed per source on reading.  A table
+    written part by part in that layout reads back as the field of the
+    Jones source, and a hand-made single-slab plane wave evaluates to its
+    closed form."""
+    from lithofem_amd.incident import (evaluate, sp_coefficients, tables_from_solve_json,
+                                       tables_to_solve_json)
+    c = contact_hole(n_xy=3, nz_scale=0.3, side=30.0, theta=6.0, phi=20.0)
+    pb = c.problem
+    bounds = [pb.slabs[0][0]] + [sl[1] for sl in pb.slabs]
+    jones = (1 / math.sqrt(2), 1j / math.sqrt(2))
+    cs, cp = sp_coefficients(6.0, 20.0, jones)
+    Ws, Wp = slab_waves(pb, 6.0, 20.0, "s", cs), slab_waves(pb, 6.0, 20.0, "p", cp)
+    parts = (tables_to_solve_json({3: Ws}, bounds) + tables_to_solve_json({3: Wp}, bounds)
+             + tables_to_solve_json({5: Ws}, bounds))
+    parts = json.loads(json.dumps(parts))
+    got = tables_from_solve_json(parts, len(pb.slabs))
+    assert sorted(got) == [3, 5]
+    want = slab_waves(pb, 6.0, 20.0, jones)
+    z = np.linspace(0.5, 119.5, 41)
+    for j, (lo, hi, _) in enumerate(pb.slabs):
+        zz = z[(z >= lo) & (z < hi)]
+        a, b = evaluate(got[3], j, zz), evaluate(want, j, zz)
+        assert np.abs(a - b).max() < 1e-14 * np.abs(b).max()
+    assert np.array_equal(got[5], Ws)
+    # one vacuum slab, E = e_s exp(-i kz (z - 120)) written by hand
+    kz = 0.3
+    hand = [{"source_index": 0, "slabs": [{"z": [0.0, 120.0], "qA": [-kz, 0.0], "zA": 120.0,
+                                           "A": [[0, 0], [1, 0], [0, 0]], "qB": [kz, 0.0],
+                                           "zB": 0.0, "B": [[0, 0], [0, 0], [0, 0]]}]}]
+    W = tables_from_solve_json(hand, 1)[0]
+    e = evaluate(W, 0, z)
+    assert np.abs(e[:, 1] - np.exp(-1j * kz * (z - 120.0))).max() < 1e-15
+    assert np.abs(e[:, [0, 2]]).max() == 0.0
+    # parts of one source with different propagation constants are refused
+    bad = json.loads(json.dumps(hand))
+    bad[0]["slabs"][0]["qA"] = [-0.31, 0.0]
+    with pytest.raises(NotImplementedError):
+        tables_from_solve_json(hand + bad, 1)
 
 
 @pytest.mark.parametrize("pol", ["s", "p"])
@@ -311,7 +484,7 @@ def test_incident_fresnel_and_energy(pol, eps2):
     incident flux (R + T = 1, lossless or absorbing substrate)."""
     pb = _stack([(0.0, 60.0, eps2), (60.0, 120.0, 1.0)])
     theta = 15.0
-    W = slab_waves(pb, theta, 0.0, (1.0, 0.0) if pol == "s" else (0.0, 1.0))
+    W = slab_waves(pb, theta, 0.0, pol)
     k0 = pb.k0
     kp = k0 * math.sin(math.radians(theta))
     q1 = np.sqrt(complex(k0 * k0 - kp * kp))
@@ -441,4 +614,20 @@ def test_solve_json_problem():
     o2.set_incident(incident_set([pb], [g.fields]))
     for r in range(2):
         assert np.array_equal(o1.rhs(r), o2.rhs(r))
+    # Jones (1, i)/sqrt 2 read as the transverse (Ex, Ey) of the incident field
     assert g.fields[1][2] == (pytest.approx(1 / np.sqrt(2)), pytest.approx(1j / np.sqrt(2)))
+    assert g.tables is None and g.incident_fields() is g.fields
+    # the group's own incident tables (the reference driver's layout) are
+    # read when present, and reproduce the recomputed RHS
+    from lithofem_amd.incident import slab_waves as sw, tables_to_solve_json
+    bounds = d["slabs"]
+    d2 = json.loads(json.dumps(d))
+    d2["groups"][0]["incident"] = tables_to_solve_json(
+        {i: sw(pb, *f) for i, f in zip(g.source_indices, g.fields)}, bounds)
+    (g2,) = load_solve_json(d2)
+    assert g2.tables is not None and len(g2.tables) == 2
+    o3 = O.Oracle(ad2, pb, I, J)
+    o3.set_incident(incident_set([pb], [g2.incident_fields()]))
+    for r in range(2):
+        want = o2.rhs(r)
+        assert np.abs(o3.rhs(r) - want).max() <= 1e-14 * np.abs(want).max()

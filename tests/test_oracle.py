@@ -96,6 +96,45 @@ def _whitney(X):
     return vol, mean, curl, T
 
 
+def test_whitney_curlcurl_linear_pml_pins_rule_point_z():
+    """z at the rule points: with pml_order = 1 the stretch s(z) is linear
+    in depth, and at p = 1 the curl-curl rule (order 0) is the centroid, so
+    cc = vol curl diag(1/s, 1/s, s)(z_c) curl^T with z_c the mean vertex z.
+    The tet is skewed so that the third row of J (z of the three edges,
+    the right map z = v0_z + J[6..8] xi) and its third column (the three
+    components of edge 3) differ: a transposed map would move z_c."""
+    import dataclasses
+    c = line_grating(nx=3, ny=3, nz_scale=0.1, theta=6.0)
+    pb = copy.deepcopy(c.problem)
+    pb.pml_order = 1.0
+    ad = build_asm_data(c.mesh, 1)
+    e = int(np.nonzero(ad.attr == 6)[0][0])              # a top-PML element
+    X = np.array([[10.0, 3.0, 121.0], [14.0, 3.5, 124.0], [11.0, 8.0, 122.5],
+                  [9.5, 4.0, 128.0]])
+    A = (X[1:] - X[0]).T                                  # columns: edges 1..3
+    assert np.linalg.det(A) > 0 and abs(A[2].sum() - A[:, 2].sum()) > 1
+    Jn, Jin, dn, vn = ad.J.copy(), ad.Jinv.copy(), ad.detJ.copy(), ad.v0.copy()
+    Jn.reshape(-1, 3, 3)[e] = A
+    Jin.reshape(-1, 3, 3)[e] = np.linalg.inv(A)
+    dn[e] = np.linalg.det(A)
+    vn.reshape(-1, 3)[e] = X[0]
+    ad = dataclasses.replace(ad, J=Jn, Jinv=Jin, detJ=dn, v0=vn)
+    I, J = csr_symbolic(ad.dof, ad.height)
+    orc = O.Oracle(ad, pb, I, J)
+    vol, _, curl, _ = _whitney(X)
+    zc = X[:, 2].mean()
+    s = 1 + 1j * pb.smax(True) * (zc - pb.z_max) / pb.pml_thick
+    assert abs(orc.s_z(zc) - s) < 1e-15
+    ref = vol * np.einsum("ki,i,li->kl", curl, np.array([1 / s, 1 / s, s]), curl)
+    got = orc.element(e, O.CURLCURL, False)
+    assert maxnorm_err(got, ref) < 1e-13, maxnorm_err(got, ref)
+    # the transposed map would evaluate s at another depth
+    z_wrong = X[0, 2] + A[:, 2].sum() / 4
+    s_w = 1 + 1j * pb.smax(True) * (z_wrong - pb.z_max) / pb.pml_thick
+    wrong = vol * np.einsum("ki,i,li->kl", curl, np.array([1 / s_w, 1 / s_w, s_w]), curl)
+    assert maxnorm_err(wrong, ref) > 1e-3
+
+
 def _kmat(kx, ky):
     return np.array([[0, 0, ky], [0, 0, -kx], [-ky, kx, 0]], dtype=float)
 
@@ -351,7 +390,7 @@ def _inc(pb, fields):
 def test_eps_sensitivity_matches_finite_differences():
     c, ad, I, J = case("grating", 2, 6.0, "tiny")
     pb = c.problem
-    inc = _inc(pb, [(6.0, 0.0, (0.0, 1.0))])
+    inc = _inc(pb, [(6.0, 0.0, "p")])
     rng = np.random.default_rng(11)
     eps0 = np.array([pb.regions[a - 1].eps for a in ad.attr], dtype=np.complex128)
     lam = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
@@ -375,6 +414,56 @@ def test_eps_sensitivity_matches_finite_differences():
                 vals.append(lam @ (_spmv(I, J, vre + 1j * vim, u) - oe.rhs(0)))
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[e] * dirn) < 1e-8 * abs(g[e]), (e, dirn, fd, g[e])
+
+
+@pytest.mark.parametrize("name,p", [("grating", 3), ("hole", 2)])
+def test_per_nonzero_metric_and_scales(name, p):
+    """The componentwise scale bounds every entry (|r_k| <= s_k), and two
+    correct evaluation orders of the same formulas -- the oracle and its
+    own -O3 FMA-contracted build -- agree to ~1e-16 of the scale at every
+    nonzero.  Their plain relative error |g - r| / |r| over the entries at
+    least 1e-8 of the largest is far larger (entries that cancel): that
+    metric measures the rounding order of small entries, which is why the
+    GPU parity tests assert it on well-conditioned nonzeros and the scaled
+    metric on all of them."""
+    from cases import pernz_err, scaled_err
+    c, ad, I, J = case(name, p, 6.0, "small")
+    o1 = O.Oracle(ad, c.problem, I, J)
+    o2 = O.Oracle(ad, c.problem, I, J, fast=True)
+    a1, a2 = o1.assemble(), o2.assemble()
+    v1, v2 = a1[0] + 1j * a1[1], a2[0] + 1j * a2[1]
+    assert not np.array_equal(v1, v2)
+    sc = o1.scale_csr(4)
+    assert np.all(np.abs(v1) <= sc * (1 + 1e-12))
+    assert maxnorm_err(v2, v1) < 1e-14
+    err_s = scaled_err(v2, v1, sc)
+    err_g = pernz_err(v2, v1)
+    print(f"{name} p={p}: scaled {err_s:.2e}, plain per-nonzero at 1e-8 gate {err_g:.2e}")
+    assert err_s < 2e-15
+    assert err_g > 50 * err_s
+    E1, E2 = o1.elements(0, 150), o2.elements(0, 150)
+    S = o1.scale_elements(np.arange(150))
+    assert np.all(np.abs(E1) <= S * (1 + 1e-12))
+    assert scaled_err(E2, E1, S) < 2e-15
+    # the scale is the modulus sum: equal to |r| for a single real product
+    # chain, e.g. the curl-curl entries of an element with constant s
+    Sc = o1.scale_elements([0], O.CURLCURL, False)[0]
+    Ec = o1.elements(0, 1, O.CURLCURL, False)[0]
+    assert np.all(np.abs(np.diag(Ec)) <= np.diag(Sc) * (1 + 1e-12))
+
+
+def test_threaded_oracle_equals_serial():
+    """The host-thread forms used at full size (element blocks; disjoint CSR
+    row ranges of the scatter) are bit-identical to the serial oracle."""
+    c, ad, I, J = case("hole", 2, 6.0, "small")
+    orc = O.Oracle(ad, c.problem, I, J)
+    A1 = orc.elements(0, ad.ne)
+    assert np.array_equal(orc.elements_threaded(5), A1)
+    s1 = orc.scatter(A1)
+    s2 = orc.scatter_threaded(A1, 7)
+    assert np.array_equal(s1[0], s2[0]) and np.array_equal(s1[1], s2[1])
+    a = orc.assemble()
+    assert np.array_equal(a[0], s1[0]) and np.array_equal(a[1], s1[1])
 
 
 def test_rhs_constant_field_equals_mass_action():
