@@ -219,6 +219,20 @@ int lfasm_download_local(lfasm_handle *h, double *Ae);
  * with X, B as 2n [re; im] host vectors. */
 int lfasm_residual(lfasm_handle *h, const double *X2n, const double *B2n, double *rel);
 
+/* lfasm_residual without allocations (no reference counterpart; for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * lfasm_residual (||A X - B|| / ||B||, X and B 2n [re; im] host vectors of
+ * len = 2n doubles; the norms are reduced on the device, so the last bits
+ * may differ), with no hipMalloc / hipFree: X, B and the partial sums live
+ * in the library's static device workspace (one per GPU, about 48 MB, held
+ * until the process ends).  n is limited to 1,572,864 dofs (LFASM_E_ARG
+ * beyond).  Every call on a device shares that device's workspace: calls
+ * that overlap on one GPU (several host threads or handles at once) race. */
+#define LFASM_RESIDUAL_STATIC_MAX_N 1572864
+int lfasm_residual_static(lfasm_handle *h, int64_t len, const double *X2n, const double *B2n,
+                          double *rel);
+
 /* n(), nnz(), ne(), nd() of the reference classes. */
 int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
 

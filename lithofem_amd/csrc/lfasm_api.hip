@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -39,8 +40,128 @@ enum {
 
 }  // namespace
 
+// ---- static workspace of lfasm_residual_static --------------------------------
+// The entry point that allocates nothing keeps its device memory in this
+// file-scope array: x and b staged split re/im (4 n doubles for n <= RS_MAX_N
+// dofs), then RS_NPART per-block partial sums (2 doubles each) and the final
+// pair.  Every GPU that loads the library holds its own copy for the life of
+// the process; its address on each device comes from hipGetSymbolAddress at
+// lfasm_create (host code never passes the array's name to a kernel).
+namespace {
+constexpr int RS_MAX_N = 1572864;            // 1.5 * 2^20 dofs
+constexpr int RS_NPART = 2048;
+constexpr size_t RS_PART_OFF = 4 * (size_t)RS_MAX_N;
+constexpr size_t RS_OUT_OFF = RS_PART_OFF + 2 * (size_t)RS_NPART;
+constexpr size_t RS_WORDS = RS_OUT_OFF + 2;
+constexpr int RS_MAX_DEVICES = 64;
+static_assert(RS_MAX_N == LFASM_RESIDUAL_STATIC_MAX_N, "workspace size and header disagree");
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[RS_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[RS_WORDS];
+
+namespace {
+
+// Per block: |A x - b|^2 and |b|^2 over a grid-stride set of rows (one row
+// per thread, ascending column order as k_spmv), tree-reduced in LDS; the
+// block's pair goes to part[2 b], part[2 b + 1].
+__global__ void __launch_bounds__(256) k_resid_partial(int n, const int *__restrict__ Ip,
+                                                       const int *__restrict__ Jc,
+                                                       const double2 *__restrict__ vals,
+                                                       const double *__restrict__ ws,
+                                                       double *__restrict__ part)
+{
+   const double *xr = ws, *xi = ws + n, *br = ws + 2 * (size_t)n, *bi = ws + 3 * (size_t)n;
+   double rn = 0.0, bn = 0.0;
+   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+   {
+      double yr = 0.0, yi = 0.0;
+      for (int k = Ip[r]; k < Ip[r + 1]; ++k)
+      {
+         const double2 a = vals[k];
+         const int c = Jc[k];
+         yr += a.x * xr[c] - a.y * xi[c];
+         yi += a.x * xi[c] + a.y * xr[c];
+      }
+      const double dr = yr - br[r], di = yi - bi[r];
+      rn += dr * dr + di * di;
+      bn += br[r] * br[r] + bi[r] * bi[r];
+   }
+   __shared__ double s_rn[256], s_bn[256];
+   s_rn[threadIdx.x] = rn;
+   s_bn[threadIdx.x] = bn;
+   __syncthreads();
+   for (int h = 128; h > 0; h >>= 1)
+   {
+      if ((int)threadIdx.x < h)
+      {
+         s_rn[threadIdx.x] += s_rn[threadIdx.x + h];
+         s_bn[threadIdx.x] += s_bn[threadIdx.x + h];
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0)
+   {
+      part[2 * blockIdx.x] = s_rn[0];
+      part[2 * blockIdx.x + 1] = s_bn[0];
+   }
+}
+
+// One block: the nb partial pairs summed in a fixed order into out[0..1].
+__global__ void __launch_bounds__(256) k_resid_final(int nb, const double *__restrict__ part,
+                                                     double *__restrict__ out)
+{
+   double rn = 0.0, bn = 0.0;
+   for (int b = threadIdx.x; b < nb; b += blockDim.x)
+   {
+      rn += part[2 * b];
+      bn += part[2 * b + 1];
+   }
+   __shared__ double s_rn[256], s_bn[256];
+   s_rn[threadIdx.x] = rn;
+   s_bn[threadIdx.x] = bn;
+   __syncthreads();
+   for (int h = 128; h > 0; h >>= 1)
+   {
+      if ((int)threadIdx.x < h)
+      {
+         s_rn[threadIdx.x] += s_rn[threadIdx.x + h];
+         s_bn[threadIdx.x] += s_bn[threadIdx.x + h];
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0)
+   {
+      out[0] = s_rn[0];
+      out[1] = s_bn[0];
+   }
+}
+
+// Device address of `scratch` on `device` (the current device), resolved
+// once per device and kept.
+hipError_t scratch_address(int device, double **out)
+{
+   *out = nullptr;
+   if (device < 0 || device >= RS_MAX_DEVICES) { return hipErrorInvalidDevice; }
+   std::lock_guard<std::mutex> lock(g_scratch_mu);
+   if (!g_scratch_ptr[device])
+   {
+      double *ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) { return e; }
+      g_scratch_ptr[device] = ptr;
+   }
+   *out = g_scratch_ptr[device];
+   return hipSuccess;
+}
+
+}  // namespace
+
 struct lfasm_handle {
    int device = 0;
+   double *scratch = nullptr;         // this device's address of the static workspace
    hipStream_t stream = nullptr;
    // captured assembly step (element matrices + scatter)
    hipGraph_t graph = nullptr;
@@ -886,6 +1007,9 @@ int lfasm_create(int device, lfasm_handle **out)
    }
    *out = h;
    LF_HIP(hipSetDevice(device));
+   // the static workspace of lfasm_residual_static: its address on this
+   // device, with the device current (once per device)
+   LF_HIP(scratch_address(device, &h->scratch));
    LF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
    LF_HIP(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, device));
    {
@@ -1642,6 +1766,48 @@ int lfasm_residual(lfasm_handle *h, const double *X2n, const double *B2n, double
       bn += B2n[i] * B2n[i] + B2n[n + i] * B2n[n + i];
    }
    *rel = bn > 0.0 ? std::sqrt(rn / bn) : 0.0;
+   return LFASM_OK;
+}
+
+int lfasm_residual_static(lfasm_handle *h, int64_t len, const double *X2n, const double *B2n,
+                          double *rel)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (!X2n || !B2n || !rel) { return h->fail(LFASM_E_ARG, "null vector"); }
+   if (len <= 0 || (len & 1)) { return h->fail(LFASM_E_ARG, "vector length %lld is not 2n", (long long)len); }
+   const int64_t n = len / 2;
+   if (n > RS_MAX_N)
+   {
+      return h->fail(LFASM_E_ARG,
+                     "n = %lld dofs exceeds the static workspace of lfasm_residual_static "
+                     "(at most %d dofs; lfasm_residual has no limit)", (long long)n, RS_MAX_N);
+   }
+   if (!h->global_built) { return h->fail(LFASM_E_STATE, "build_global first"); }
+   if (n != h->height)
+   {
+      return h->fail(LFASM_E_ARG, "vector length 2 x %lld does not match the system (n = %d)",
+                     (long long)n, h->height);
+   }
+   if (!h->scratch) { return h->fail(LFASM_E_STATE, "static workspace not resolved on this device"); }
+   LF_HIP(hipSetDevice(h->device));
+   double *ws = h->scratch;
+   // [x_re; x_im] and [b_re; b_im] as the caller holds them
+   LF_HIP(hipMemcpyAsync(ws, X2n, (size_t)len * sizeof(double), hipMemcpyHostToDevice, h->stream));
+   LF_HIP(hipMemcpyAsync(ws + len, B2n, (size_t)len * sizeof(double), hipMemcpyHostToDevice,
+                         h->stream));
+   // every partial the final kernel reads is written by this call: nothing
+   // in the workspace needs clearing
+   const int nb = (int)std::min<int64_t>((n + 255) / 256, RS_NPART);
+   hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)nb), dim3(256), 0, h->stream, (int)n, h->Ip,
+                      h->Jc, h->vals, (const double *)ws, ws + RS_PART_OFF);
+   LF_HIP(hipGetLastError());
+   hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, h->stream, nb,
+                      (const double *)(ws + RS_PART_OFF), ws + RS_OUT_OFF);
+   LF_HIP(hipGetLastError());
+   double out[2] = {0.0, 0.0};
+   LF_HIP(hipMemcpyAsync(out, ws + RS_OUT_OFF, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+   LF_HIP(hipStreamSynchronize(h->stream));
+   *rel = out[1] > 0.0 ? std::sqrt(out[0] / out[1]) : 0.0;
    return LFASM_OK;
 }
 

@@ -26,6 +26,9 @@ MASS_ALL = MASS | CROSS0 | CROSS1 | KLK
 
 E_ARG, E_HIP, E_FORCED, E_UNSUPPORTED, E_STATE, E_PATTERN = -1, -2, -3, -4, -5, -6
 
+# most dofs lfasm_residual_static's static workspace holds
+RESIDUAL_STATIC_MAX_N = 1572864
+
 # kernel selector of lfasm_compute_local_kernel
 KERNEL_AUTO, KERNEL_VALU = 0, 1
 N_MFMA_VARIANTS = 6
@@ -101,6 +104,7 @@ def lib():
         "lfasm_download_values": ([vp, vp, vp], i),
         "lfasm_download_local": ([vp, vp], i),
         "lfasm_residual": ([vp, vp, vp, vp], i),
+        "lfasm_residual_static": ([vp, i64, vp, vp, vp], i),
         "lfasm_sizes": ([vp, vp, vp, vp, vp], i),
         "lfasm_timings": ([vp, vp, i], i),
         "lfasm_stream": ([vp], vp),
@@ -320,6 +324,16 @@ class AsmGpuGlobal:
         B2n = np.ascontiguousarray(B2n, dtype=np.float64)
         rel = ct.c_double()
         self._check(lib().lfasm_residual(self._h, _p(X2n), _p(B2n), ct.byref(rel)))
+        return rel.value
+
+    def residual_static(self, X2n, B2n) -> float:
+        """lfasm_residual without device allocations (static workspace)."""
+        X2n = np.ascontiguousarray(X2n, dtype=np.float64)
+        B2n = np.ascontiguousarray(B2n, dtype=np.float64)
+        assert X2n.shape == B2n.shape
+        rel = ct.c_double()
+        self._check(lib().lfasm_residual_static(self._h, X2n.size, _p(X2n), _p(B2n),
+                                                ct.byref(rel)))
         return rel.value
 
     # -- eps variants (delta assembly)

@@ -100,6 +100,30 @@ def test_library_rejects_repeated_dof_in_an_element(lib):
         lib.lfasm_destroy(h)
 
 
+def test_library_residual_static_limit(lib):
+    """lfasm_residual_static checks the vector length against its static
+    workspace before it needs the device; the limit is the header's."""
+    import ctypes as ct
+    hdr = (ROOT / "include" / "lithofem_asm.h").read_text()
+    m = re.search(r"#define LFASM_RESIDUAL_STATIC_MAX_N (\d+)", hdr)
+    assert m and int(m.group(1)) == gpu.RESIDUAL_STATIC_MAX_N
+    h = ct.c_void_p()
+    lib.lfasm_create(0, ct.byref(h))          # no device here
+    try:
+        x = np.zeros(4)
+        rel = ct.c_double()
+        big = 2 * (gpu.RESIDUAL_STATIC_MAX_N + 1)
+        assert lib.lfasm_residual_static(h, big, x.ctypes.data, x.ctypes.data,
+                                         ct.byref(rel)) == gpu.E_ARG
+        assert "exceeds the static workspace" in lib.lfasm_last_error(h).decode()
+        assert lib.lfasm_residual_static(h, 3, x.ctypes.data, x.ctypes.data,
+                                         ct.byref(rel)) == gpu.E_ARG
+        assert lib.lfasm_residual_static(h, 4, x.ctypes.data, x.ctypes.data,
+                                         ct.byref(rel)) == gpu.E_STATE
+    finally:
+        lib.lfasm_destroy(h)
+
+
 def test_library_pml_smax(lib):
     import ctypes as ct
     f = lib.lfasm_pml_smax
