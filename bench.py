@@ -15,9 +15,16 @@ Prints one JSON line.  `value` is elements/s of the whole step; `roofline`
 is the kernel with the largest share of the step against its own roofline
 (HBM bytes or canonical FP64 FLOPs), every kernel's figures are under
 `detail.kernels`; `cpu_baseline` is the C oracle (the restated reference
-reassembler) timed on one host core over a bounded sample, and
-`detail.cpu_baseline_threads` the same oracle over all usable host threads
-(at most 16).
+reassembler, gcc -O3 build) doing the step's work -- local matrices, face
+transform, CSR scatter, elimination -- on one host core over a bounded
+sample, with the GPU's element matrices of part of that sample checked
+against the oracle's parity build; `detail.cpu_baseline_threads` is the same
+work on the whole mesh over all usable host threads (at most 16).  Unless
+--no-extras, `detail.variants_speedup` (64 eps variants in one delta
+assembly against 64 sequential plain assemblies, order 2, 271k tets) and
+`detail.adjoint_ratio` (forward + adjoint eps gradient against the forward
+assembly + host solve, with one central-difference check) carry the other
+two parts of the metric.
 """
 
 from __future__ import annotations
@@ -61,43 +68,87 @@ def _stdout_to_stderr():
         os.close(saved)
 
 
+def _host_threads() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def cpu_baseline(wl, I, J, seconds: float = 12.0) -> dict:
-    """Oracle reassembly (local matrices + element-order CSR scatter) of a
-    contiguous element sample, one core."""
+    """The oracle (the restated reference reassembler, -O3 FMA build) doing
+    the work of one GPU step on a contiguous element sample, one core: local
+    matrices + face transform, their element-order CSR scatter, and the
+    sample's share of the essential elimination (timed over the whole CSR,
+    times sample / ne)."""
     from oracle import oracle as O
     ad = wl.ad
-    orc = O.Oracle(ad, wl.cell.problem, I, J)
+    orc = O.Oracle(ad, wl.cell.problem, I, J, fast=True)
     # calibrate on a small block, then size the sample to ~`seconds`
     t0 = time.perf_counter()
     orc.elements(0, 64)
     per = (time.perf_counter() - t0) / 64
     n = int(min(ad.ne, max(256, seconds / per)))
     e0 = (ad.ne - n) // 2
+    nnz = len(J)
+    vre, vim = np.zeros(nnz), np.zeros(nnz)
     t0 = time.perf_counter()
     Ae = orc.elements(e0, e0 + n)
-    t_local = time.perf_counter() - t0
-    return dict(value=n / t_local, unit="elements/s", cores=1, kind="port",
-                sample=f"elements [{e0}, {e0 + n}) of the same mesh ({n} of {ad.ne}), "
-                       f"oracle local matrices + face transform, {t_local:.1f} s",
-                _Ae=Ae, _e0=e0, _orc=orc, _per=t_local / n)
+    t1 = time.perf_counter()
+    orc.scatter_elements_into(Ae, e0, vre, vim)
+    t2 = time.perf_counter()
+    is_ess = np.zeros(ad.height, np.uint8)
+    is_ess[ad.ess] = 1
+    t3 = time.perf_counter()
+    O.eliminate(I, J, is_ess, vre, vim)
+    t_elim = time.perf_counter() - t3
+    t_total = (t1 - t0) + (t2 - t1) + t_elim * n / ad.ne
+    return dict(value=n / t_total, unit="elements/s", cores=1, kind="port",
+                sample=(f"elements [{e0}, {e0 + n}) of the same mesh ({n} of {ad.ne}): oracle "
+                        f"local matrices + face transform {t1 - t0:.1f} s, their CSR scatter "
+                        f"{t2 - t1:.2f} s, elimination share {t_elim * n / ad.ne:.3f} s "
+                        f"(gcc -O3 -march=x86-64-v3 -ffp-contract=fast)"),
+                _Ae=Ae, _e0=e0, _per=t_total / n)
 
 
-def cpu_baseline_threads(cb: dict, ne: int, seconds: float = 6.0) -> dict:
-    """The same oracle over all host threads the process may use (at most 16):
-    one contiguous element block per thread (each thread writes its own
-    output; the C calls release the interpreter lock)."""
-    from concurrent.futures import ThreadPoolExecutor
-    nt = max(1, min(16, len(os.sched_getaffinity(0))))
-    orc = cb["_orc"]
-    n = int(min(ne, max(nt * 64, nt * seconds / cb["_per"])))
-    e0 = (ne - n) // 2
-    cuts = [e0 + (n * i) // nt for i in range(nt + 1)]
-    with ThreadPoolExecutor(nt) as ex:
-        t0 = time.perf_counter()
-        list(ex.map(lambda i: orc.elements(cuts[i], cuts[i + 1]), range(nt)))
-        t = time.perf_counter() - t0
-    return dict(value=n / t, unit="elements/s", cores=nt, kind="port",
-                sample=f"elements [{e0}, {e0 + n}) in {nt} contiguous blocks, {t:.1f} s")
+def cpu_baseline_threads(wl, I, J) -> dict:
+    """The same work on the whole mesh over all host threads the process may
+    use (at most 16): element blocks, then disjoint CSR row ranges of the
+    scatter, then the elimination (one thread)."""
+    from oracle import oracle as O
+    ad = wl.ad
+    nt = _host_threads()
+    orc = O.Oracle(ad, wl.cell.problem, I, J, fast=True)
+    is_ess = np.zeros(ad.height, np.uint8)
+    is_ess[ad.ess] = 1
+    t0 = time.perf_counter()
+    Ae = orc.elements_threaded(nt)
+    t1 = time.perf_counter()
+    vre, vim = orc.scatter_threaded(Ae, nt)
+    del Ae
+    t2 = time.perf_counter()
+    O.eliminate(I, J, is_ess, vre, vim)
+    t3 = time.perf_counter()
+    return dict(value=ad.ne / (t3 - t0), unit="elements/s", cores=nt, kind="port",
+                sample=(f"all {ad.ne} elements: local matrices {t1 - t0:.1f} s ({nt} element "
+                        f"blocks), CSR scatter {t2 - t1:.1f} s ({nt} row ranges), elimination "
+                        f"{t3 - t2:.2f} s"))
+
+
+def sample_parity(wl, I, J, Ag, e0: int, n: int = 192) -> dict:
+    """GPU element matrices of `n` elements of the baseline sample against the
+    parity build of the oracle: max-normalised error, error per entry
+    against the entries' componentwise scales, and plain relative error per
+    entry over the entries >= 1e-8 of the largest."""
+    from oracle import oracle as O
+    orc = O.Oracle(wl.ad, wl.cell.problem, I, J)
+    ref = orc.elements(e0, e0 + n)
+    sc = orc.scale_elements(np.arange(e0, e0 + n))
+    got = Ag[e0:e0 + n]
+    d = np.abs(got - ref)
+    a = np.abs(ref)
+    gate = a >= 1e-8 * a.max()
+    return dict(elements=f"[{e0}, {e0 + n})",
+                maxnorm=float(d.max() / a.max()),
+                per_entry_vs_scale=float(np.max(d[sc > 0] / sc[sc > 0])),
+                per_entry_relative_gated=float(np.max(d[gate] / a[gate])))
 
 
 def main() -> int:
@@ -115,8 +166,11 @@ def main() -> int:
     ap.add_argument("--tensor", default="on", choices=["on", "off", "fused"],
                     help="constant-coefficient elements: reference-tensor kernel (on), "
                          "quadrature kernels (off), tensor rows inside the scatter (fused)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-variant and adjoint figures of the default run")
     ap.add_argument("--mode", default="assemble",
-                    choices=["assemble", "multimesh", "variants", "rhs", "adjoint", "large"],
+                    choices=["assemble", "multimesh", "variants", "rhs", "adjoint", "large",
+                             "peaks"],
                     help="assemble: the headline step; others: batched / adjoint workloads")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -129,6 +183,11 @@ def main() -> int:
     from lithofem_amd.workload import (canonical_flops, line_grating_p3, local_bytes,
                                        scatter_bytes, tensor_bytes)
 
+    if args.mode == "peaks":
+        out = measure_peaks()
+        out.update({"n_gpus": 1, "mode": "peaks"})
+        print(json.dumps(out), flush=True)
+        return 0
     if args.mode != "assemble":
         from lithofem_amd import benchmarks
         out = getattr(benchmarks, args.mode)(log)
@@ -313,22 +372,56 @@ def main() -> int:
     }
     if not args.no_cpu:
         cb = cpu_baseline(wl, I, J)
-        # the GPU element matrices of the sample match the oracle's
         with _stdout_to_stderr():
             Ag = g.download_local()
-        n = cb["_Ae"].shape[0]
-        ref = cb["_Ae"]
-        err = float(np.max(np.abs(Ag[cb["_e0"]:cb["_e0"] + n] - ref)) / np.max(np.abs(ref)))
-        log(f"cpu baseline {cb['value']:.1f} elements/s; sample parity {err:.2e}")
+        par = sample_parity(wl, I, J, Ag, cb["_e0"])
+        del Ag
+        log(f"cpu baseline {cb['value']:.1f} elements/s ({cb['sample']}); sample parity {par}")
         out["cpu_baseline"] = {k: v for k, v in cb.items() if not k.startswith("_")}
-        out["cpu_baseline"]["sample_parity_maxnorm"] = err
+        out["cpu_baseline"]["sample_parity"] = par
         out["cpu_baseline"]["host_cpu"] = _cpu_model()
-        ct = cpu_baseline_threads(cb, ad.ne)
-        log(f"cpu baseline, {ct['cores']} threads: {ct['value']:.1f} elements/s")
-        out["detail"]["cpu_baseline_threads"] = ct
     g.close()
+    if not args.no_cpu:
+        ct = cpu_baseline_threads(wl, I, J)
+        log(f"cpu baseline, {ct['cores']} threads: {ct['value']:.1f} elements/s ({ct['sample']})")
+        out["detail"]["cpu_baseline_threads"] = ct
+    if not args.no_extras:
+        from lithofem_amd import benchmarks
+        v = benchmarks.variants(log)
+        out["detail"]["variants_speedup"] = v["value"]
+        out["detail"]["variants"] = v["detail"]
+        a = benchmarks.adjoint(log, n_fd=1, region_check=False)
+        out["detail"]["adjoint_ratio"] = a["value"]
+        out["detail"]["adjoint"] = a["detail"]
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measure_peaks(device: int = 0) -> dict:
+    """Device ceilings from lithofem_amd/libpeaks.so (the probe library the
+    assembly library does not contain): FP64 MFMA over chains per wave and
+    waves per SIMD with the shader clock under load, FP64 VALU, HBM
+    streams."""
+    import ctypes as ct
+
+    import torch  # noqa: F401  (one HIP runtime per process)
+    L = ct.CDLL(str(ROOT / "lithofem_amd" / "libpeaks.so"))
+    L.lfpeaks_measure.argtypes = [ct.c_int, ct.c_void_p, ct.c_int]
+    L.lfpeaks_measure.restype = ct.c_int
+    out = np.zeros(11 + 4 * 16)
+    rows = L.lfpeaks_measure(device, out.ctypes.data, len(out))
+    if rows < 0:
+        raise RuntimeError(f"lfpeaks_measure failed ({rows})")
+    sweep = [dict(chains=int(out[11 + 4 * i]), waves_per_simd=int(out[12 + 4 * i]),
+                  tflops=out[13 + 4 * i], clock_ghz=out[14 + 4 * i]) for i in range(rows)]
+    cus = int(out[9])
+    return {"fp64_mfma_tflops": out[0], "fp64_mfma_clock_ghz": out[1],
+            "fp64_mfma_best": {"chains": int(out[2]), "waves_per_simd": int(out[3])},
+            "fp64_mfma_at_clock_peak_tflops": cus * 4 * 32 * out[1] * 1e9 / 1e12,
+            "fp64_valu_tflops": out[4], "fp64_valu_clock_ghz": out[5],
+            "fp64_valu_at_clock_peak_tflops": cus * 4 * 32 * out[5] * 1e9 / 1e12,
+            "hbm_copy_gbps": out[6], "hbm_write_gbps": out[7], "hbm_read_gbps": out[8],
+            "cus": cus, "reported_max_clock_ghz": out[10], "mfma_sweep": sweep}
 
 
 def _quad_kernel_name(selector: int, nd: int) -> str:

@@ -309,15 +309,6 @@ void *lfasm_stream(lfasm_handle *h);
 int64_t lfasm_csr_symbolic(int ne, int nd, int height, const int *dof, int *I, int **Jcol);
 void lfasm_free(void *p);
 
-/* Measured device ceilings (no reference counterpart): out[0] FP64 MFMA
- * TFLOP/s (v_mfma_f64_16x16x4f64 loop), out[1] FP64 VALU FMA TFLOP/s,
- * out[2] HBM copy GB/s (2 GiB read + 2 GiB write), out[3] reported clock
- * GHz, out[4] compute units, out[5] HBM write-only GB/s (2 GiB), out[6] HBM
- * read-only GB/s (2 GiB); each stream figure is the best over launch shapes
- * (1,024 to 16,384 blocks of 256 threads, one or four accesses per thread)
- * and ordinary or non-temporal accesses.  `out` holds 7 doubles.  Allocates
- * 4 GiB of device memory while it runs. */
-int lfasm_measure_peaks(int device, double *out);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,6 @@
 // one HIP stream on its device.  Errors are returned as status codes with a
 // message string instead of printf + bool (solver/asm_gpu.inc:255-263).
 #include "lfasm_device.hpp"
-#include "lfasm_peaks.hpp"
 #include "lfasm_batch.hpp"
 #include "lfasm_tensor.hpp"
 #include "lfasm_pattern.hpp"
@@ -2412,113 +2411,5 @@ int64_t lfasm_csr_symbolic(int ne, int nd, int height, const int *dof, int *I, i
 }
 
 void lfasm_free(void *p) { std::free(p); }
-
-int lfasm_measure_peaks(int device, double *out)
-{
-   if (!out) { return LFASM_E_ARG; }
-   if (hipSetDevice(device) != hipSuccess) { return LFASM_E_HIP; }
-   hipDeviceProp_t prop;
-   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { return LFASM_E_HIP; }
-   const int ncu = prop.multiProcessorCount;
-   hipStream_t s = nullptr;
-   hipEvent_t e0 = nullptr, e1 = nullptr;
-   double *dout = nullptr;
-   double2 *a = nullptr, *b = nullptr;
-   const size_t nbytes = (size_t)1 << 31;   // 2 GiB per buffer
-   const size_t n2 = nbytes / sizeof(double2);
-   int rc = LFASM_OK;
-   float ms = 0.0f;
-   auto ok = [&](hipError_t e) { if (e != hipSuccess && rc == LFASM_OK) { rc = LFASM_E_HIP; } return rc == LFASM_OK; };
-   if (!ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) || !ok(hipEventCreate(&e0)) ||
-       !ok(hipEventCreate(&e1)) || !ok(hipMalloc((void **)&dout, 65536 * sizeof(double))) ||
-       !ok(hipMalloc((void **)&a, nbytes)) || !ok(hipMalloc((void **)&b, nbytes)))
-   {
-      goto done;
-   }
-   {
-      // FP64 MFMA: 8 waves per CU x 4 blocks per CU
-      const int iters = 4000;
-      const int grid = ncu * 8;
-      hipLaunchKernelGGL(k_peak_mfma, dim3(grid), dim3(256), 0, s, 10, dout);
-      ok(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(k_peak_mfma, dim3(grid), dim3(256), 0, s, iters, dout);
-      ok(hipEventRecord(e1, s));
-      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
-      const double flops = (double)grid * 4 * iters * 8 * (2.0 * 16 * 16 * 4);
-      out[0] = flops / (ms * 1e-3) / 1e12;
-   }
-   {
-      const int iters = 20000;
-      const int grid = ncu * 8;
-      hipLaunchKernelGGL(k_peak_valu, dim3(grid), dim3(256), 0, s, 10, 0.999999, dout);
-      ok(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(k_peak_valu, dim3(grid), dim3(256), 0, s, iters, 0.999999, dout);
-      ok(hipEventRecord(e1, s));
-      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
-      const double flops = (double)grid * 256 * iters * 8 * 2.0;
-      out[1] = flops / (ms * 1e-3) / 1e12;
-   }
-   {
-      hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(256), 0, s, a, n2);
-      hipLaunchKernelGGL(k_peak_copy, dim3(ncu * 16), dim3(256), 0, s, a, b, n2);
-      ok(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(k_peak_copy, dim3(ncu * 16), dim3(256), 0, s, a, b, n2);
-      ok(hipEventRecord(e1, s));
-      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
-      out[2] = 2.0 * (double)nbytes / (ms * 1e-3) / 1e9;
-      out[3] = (double)prop.clockRate / 1e6;   // GHz (reported maximum)
-      out[4] = ncu;
-   }
-   {
-      hipLaunchKernelGGL(k_peak_write, dim3(ncu * 32), dim3(256), 0, s, b, n2);
-      ok(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(k_peak_write, dim3(ncu * 32), dim3(256), 0, s, b, n2);
-      ok(hipEventRecord(e1, s));
-      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
-      out[5] = (double)nbytes / (ms * 1e-3) / 1e9;
-      hipLaunchKernelGGL(k_peak_read, dim3(ncu * 32), dim3(256), 0, s, a, n2, dout);
-      ok(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(k_peak_read, dim3(ncu * 32), dim3(256), 0, s, a, n2, dout);
-      ok(hipEventRecord(e1, s));
-      if (!ok(hipEventSynchronize(e1)) || !ok(hipEventElapsedTime(&ms, e0, e1))) { goto done; }
-      out[6] = (double)nbytes / (ms * 1e-3) / 1e9;
-   }
-   // launch-shape and store-mode variants: keep the best of each stream
-   {
-      const pk_dv2 *va = reinterpret_cast<const pk_dv2 *>(a);
-      pk_dv2 *vb = reinterpret_cast<pk_dv2 *>(b);
-      auto timed = [&](auto launch, double bytes, double &best) -> bool {
-         launch();
-         if (!ok(hipEventRecord(e0, s))) { return false; }
-         launch();
-         if (!ok(hipEventRecord(e1, s)) || !ok(hipEventSynchronize(e1)) ||
-             !ok(hipEventElapsedTime(&ms, e0, e1))) { return false; }
-         best = std::max(best, bytes / (ms * 1e-3) / 1e9);
-         return true;
-      };
-      for (int g : {4, 16, 64})
-      {
-         const dim3 grid((unsigned)(ncu * g));
-         const double nb = (double)nbytes;
-         if (!timed([&] { hipLaunchKernelGGL((k_peak_copy_v<1, false>), grid, dim3(256), 0, s, va, vb, n2); }, 2 * nb, out[2]) ||
-             !timed([&] { hipLaunchKernelGGL((k_peak_copy_v<4, true>), grid, dim3(256), 0, s, va, vb, n2); }, 2 * nb, out[2]) ||
-             !timed([&] { hipLaunchKernelGGL((k_peak_write_v<4, false>), grid, dim3(256), 0, s, vb, n2); }, nb, out[5]) ||
-             !timed([&] { hipLaunchKernelGGL((k_peak_write_v<4, true>), grid, dim3(256), 0, s, vb, n2); }, nb, out[5]) ||
-             !timed([&] { hipLaunchKernelGGL((k_peak_read_v<4, false>), grid, dim3(256), 0, s, va, n2, dout); }, nb, out[6]) ||
-             !timed([&] { hipLaunchKernelGGL((k_peak_read_v<4, true>), grid, dim3(256), 0, s, va, n2, dout); }, nb, out[6]))
-         {
-            goto done;
-         }
-      }
-   }
-done:
-   if (a) { (void)hipFree(a); }
-   if (b) { (void)hipFree(b); }
-   if (dout) { (void)hipFree(dout); }
-   if (e0) { (void)hipEventDestroy(e0); }
-   if (e1) { (void)hipEventDestroy(e1); }
-   if (s) { (void)hipStreamDestroy(s); }
-   return rc;
-}
 
 }  // extern "C"

@@ -110,7 +110,6 @@ def lib():
         "lfasm_stream": ([vp], vp),
         "lfasm_csr_symbolic": ([i, i, i, vp, vp, ct.POINTER(vp)], i64),
         "lfasm_free": ([vp], None),
-        "lfasm_measure_peaks": ([i, vp], i),
         "lfasm_init_batch": ([vp, vp, i], i),
         "lfasm_batch_offsets": ([vp, vp, vp], i),
         "lfasm_variants_prepare": ([vp, i, vp], i),
@@ -151,16 +150,6 @@ def csr_symbolic(dof: np.ndarray, height: int):
     J = np.ctypeslib.as_array(ct.cast(pj, ct.POINTER(ct.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
     lib().lfasm_free(pj)
     return I, J
-
-
-def measure_peaks(device: int = 0) -> dict:
-    out = np.zeros(7)
-    rc = lib().lfasm_measure_peaks(device, _p(out))
-    if rc != 0:
-        raise AsmGpuError(rc, "measure_peaks failed")
-    return dict(fp64_mfma_tflops=out[0], fp64_valu_tflops=out[1], hbm_copy_gbps=out[2],
-                clock_ghz=out[3], cus=int(out[4]), hbm_write_gbps=out[5],
-                hbm_read_gbps=out[6])
 
 
 class AsmGpuGlobal:

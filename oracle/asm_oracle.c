@@ -351,6 +351,21 @@ int orc_scatter(const orc_asm_data *ad, const double *Ae_all, double *vre, doubl
    return 0;
 }
 
+int orc_scatter_elements(const orc_asm_data *ad, const double *Ae_range, int e0, int e1,
+                         double *vre, double *vim)
+{
+   const size_t n2 = (size_t)ad->nd * ad->nd;
+   for (int e = e0; e < e1; ++e)
+   {
+      if (scatter_one(ad, e, (const C *)(Ae_range + (size_t)(e - e0) * n2 * 2), 0, ad->height,
+                      vre, vim))
+      {
+         return -1;
+      }
+   }
+   return 0;
+}
+
 int orc_scatter_rows(const orc_asm_data *ad, const double *Ae_all, int r0, int r1, double *vre,
                      double *vim)
 {

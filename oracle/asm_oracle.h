@@ -82,6 +82,12 @@ int orc_reassemble_all(const orc_asm_data *ad, const orc_problem *pb,
 int orc_scatter(const orc_asm_data *ad, const double *Ae_all,
                 double *vre, double *vim);
 
+/* scatter of the local matrices of elements [e0, e1) (Ae_range holds just
+ * those, element-major), added to vre / vim in element order (the CPU
+ * baseline's sample) */
+int orc_scatter_elements(const orc_asm_data *ad, const double *Ae_range, int e0, int e1,
+                         double *vre, double *vim);
+
 /* orc_scatter restricted to the CSR rows [r0, r1): zeroes and fills only
  * their values, each entry accumulated in the same ascending element order
  * (disjoint row ranges may run on separate host threads; their union is

@@ -79,6 +79,9 @@ def lib(fast: bool = False):
         L.orc_scatter_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
                                        ct.c_void_p, ct.c_void_p]
         L.orc_scatter_rows.restype = ct.c_int
+        L.orc_scatter_elements.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                           ct.c_void_p, ct.c_void_p]
+        L.orc_scatter_elements.restype = ct.c_int
         L.orc_scale_range.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int,
                                       ct.c_int, ct.c_void_p]
         L.orc_scale_scatter_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
@@ -180,6 +183,16 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("CSR position missing")
         return vre, vim
+
+    def scatter_elements_into(self, Ae_range: np.ndarray, e0: int, vre: np.ndarray,
+                              vim: np.ndarray):
+        """Add the local matrices of elements [e0, e0 + len(Ae_range)) into
+        vre / vim (float64 arrays of nnz, accumulated in element order)."""
+        Ae_range = np.ascontiguousarray(Ae_range, dtype=np.complex128)
+        rc = self.L.orc_scatter_elements(ct.byref(self.data), _ptr(Ae_range), e0,
+                                         e0 + Ae_range.shape[0], _ptr(vre), _ptr(vim))
+        if rc != 0:
+            raise RuntimeError("CSR position missing")
 
     def elements_threaded(self, nthreads: int, mask: int = ALL, apply_te: bool = True):
         """All element matrices, contiguous element blocks on host threads
