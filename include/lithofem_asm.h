@@ -104,6 +104,21 @@ double lfasm_pml_smax(double target, double order, double k0, double n_med, doub
  * LITHOFEM_ASM_FORCE_FAIL (returns LFASM_E_FORCED before any device call). */
 int lfasm_init(lfasm_handle *h, const lfasm_mesh *m);
 
+/* AsmGpu::Init with the element geometry extracted on the device
+ * (solver/asm_data.inc:152-171 reads J, detJ, J^-1 and x(0) from MFEM's
+ * element transformation on the host): elem_xyz holds every element's four
+ * vertex coordinates (ne x 4 x 3, the element's local vertex order,
+ * unwrapped across periodic boundaries), from which the device forms
+ * J = [x1 - x0, x2 - x0, x3 - x0] (row-major), detJ (cofactor expansion of
+ * the first row), J^-1 = adj(J) / detJ and v0 = x0.  m's J, Jinv, detJ, v0 are
+ * not read (may be NULL); the other fields as lfasm_init.  An element with
+ * detJ <= 0 is refused (LFASM_E_ARG). */
+int lfasm_init_vertices(lfasm_handle *h, const lfasm_mesh *m, const double *elem_xyz);
+
+/* Host copies of the element geometry the handle holds (ne x 9, ne x 9, ne,
+ * ne x 3); any pointer may be NULL. */
+int lfasm_download_geometry(lfasm_handle *h, double *J, double *Jinv, double *detJ, double *v0);
+
 /* Batched multi-mesh init (no reference counterpart): the meshes (same
  * order, same rule tables and TInv; each with its own geometry, region
  * table, PML and k_par) are assembled as one block-diagonal system whose
@@ -138,6 +153,12 @@ int lfasm_build_global_device(lfasm_handle *h, const unsigned char *is_ess);
 
 /* Host copy of the CSR pattern (I: height+1, Jcol: nnz); either may be NULL. */
 int lfasm_download_pattern(lfasm_handle *h, int *I, int *Jcol);
+
+/* Host copy of the per-row contributor lists of the scatter (built on the
+ * device by both build_global calls): rc_ptr height+1, rc_list ne*nd entries
+ * e*nd + a of the elements holding each row's dof, ascending; either may be
+ * NULL. */
+int lfasm_download_contributors(lfasm_handle *h, int *rc_ptr, int *rc_list);
 
 /* AsmGpu::ComputeLocal (solver/asm_gpu.inc:326-351): element matrices of
  * the CurlCurlIntegrator (mask_cc & LFASM_CURLCURL) on the 2p-2 rule plus
@@ -243,7 +264,7 @@ int lfasm_sizes(const lfasm_handle *h, int *ne, int *nd, int *n, int64_t *nnz);
  * [5] reference-tensor part and [6] quadrature part of the last
  * compute_local (0 when that part did not run), [7] last lfasm_step_launch,
  * [8] lfasm_build_global_device's pattern phase (both kernel passes and the
- * row-pointer scan between them). */
+ * row-pointer scan between them), [9] lfasm_init_vertices' geometry kernel. */
 int lfasm_timings(const lfasm_handle *h, double *ms, int n);
 
 /* ---- eps variants (no reference counterpart) ------------------------------

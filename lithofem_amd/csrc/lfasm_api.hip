@@ -9,9 +9,11 @@
 #include "lfasm_tensor.hpp"
 #include "lfasm_pattern.hpp"
 #include "lfasm_sym.hpp"
+#include "lfasm_extract.hpp"
 #include "../../include/lithofem_asm.h"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdarg>
@@ -34,7 +36,7 @@ struct DevRule {
 
 enum {
    T_LOCAL = 0, T_SCATTER = 1, T_BUILD = 2, T_BATCH = 3, T_ADJ = 4,
-   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_STEP = 7, T_PATTERN = 8, T_N = 9
+   T_LOCAL_TENSOR = 5, T_LOCAL_QUAD = 6, T_STEP = 7, T_PATTERN = 8, T_GEOM = 9, T_N = 10
 };
 
 }  // namespace
@@ -203,6 +205,7 @@ struct lfasm_handle {
    double *coef = nullptr;            // ne x TEN_NC element coefficients (fused mode)
    // fused row units per local-index type (edge, face, interior)
    bool fused_ok = false;
+   bool fused_tried = false;          // the fused units were attempted for this pattern
    UnitHdr *units[3] = {nullptr, nullptr, nullptr};
    int *rc_enc = nullptr;             // contributor entries of the fused kernels
    std::vector<unsigned char> eclass_host;
@@ -350,6 +353,7 @@ void release_global(lfasm_handle *h)
    }
    dfree(h->rc_enc);
    h->fused_ok = false;
+   h->fused_tried = false;
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    dfree(h->rsoff); dfree(h->rslot);
    dfree(h->emask);
@@ -478,16 +482,16 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
 
 // Symmetric-split kernel (nd <= 48 with 3 tiles; CROSS0 and CROSS1 both on
 // or both off).
-template <int NBUF, int WPEU>
+template <int NBUF, int WPEU, bool PRIO = false>
 hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
 {
    const int nqt = P.rc.nq + P.rm.nq;
    const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + NBUF * SYM_STG) * sizeof(double);
-   hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_sym<NBUF, WPEU>,
+   hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_sym<NBUF, WPEU, PRIO>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   hipLaunchKernelGGL((k_local_mfma_sym<NBUF, WPEU>), dim3((unsigned)P.nout), dim3(SYM_THREADS),
-                      shmem, h->stream, P, nqt);
+   hipLaunchKernelGGL((k_local_mfma_sym<NBUF, WPEU, PRIO>), dim3((unsigned)P.nout),
+                      dim3(SYM_THREADS), shmem, h->stream, P, nqt);
    return hipGetLastError();
 }
 
@@ -518,11 +522,11 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
    case 2 * 16 + 4: return launch_mfma_pipe<2, 2, 2>(h, P);
    case 2 * 16 + 5: return launch_mfma_pipe<2, 1, 2>(h, P);
    case 3 * 16 + 0: return sym_applies(P) ? launch_mfma_sym<1, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 1: return launch_mfma_pipe<3, 2, 2>(h, P);
+   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<1, 4>(h, P) : launch_mfma_pipe<3, 2, 2>(h, P);
    case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
    case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<2, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 5: return launch_mfma_pipe<3, 1, 2>(h, P);
+   case 3 * 16 + 5: return sym_applies(P) ? launch_mfma_sym<1, 3, true>(h, P) : launch_mfma_pipe<3, 1, 2>(h, P);
    case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
    case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
@@ -1053,7 +1057,9 @@ double lfasm_pml_smax(double target, double order, double k0, double n_med, doub
 
 namespace {
 
-int validate_mesh(lfasm_handle *h, const lfasm_mesh *m, int idx)
+// geometry: the mesh carries J, Jinv, detJ, v0 (else they come from vertex
+// coordinates on the device and are checked there)
+int validate_mesh(lfasm_handle *h, const lfasm_mesh *m, int idx, bool geometry = true)
 {
    if (m->ne <= 0 || m->nd <= 0 || m->p <= 0 || m->height <= 0 || m->nreg <= 0)
    {
@@ -1064,8 +1070,8 @@ int validate_mesh(lfasm_handle *h, const lfasm_mesh *m, int idx)
       return h->fail(LFASM_E_UNSUPPORTED, "mesh %d: order %d / nd %d outside the kernels (p <= 4)",
                      idx, m->p, m->nd);
    }
-   if (!m->dof || !m->sign || !m->attr || !m->J || !m->Jinv || !m->detJ || !m->v0 ||
-       !m->eps_tab || !m->TInv || (m->p >= 2 && !m->fo))
+   if (!m->dof || !m->sign || !m->attr || !m->eps_tab || !m->TInv || (m->p >= 2 && !m->fo) ||
+       (geometry && (!m->J || !m->Jinv || !m->detJ || !m->v0)))
    {
       return h->fail(LFASM_E_ARG, "mesh %d: missing mesh array", idx);
    }
@@ -1107,7 +1113,7 @@ int validate_mesh(lfasm_handle *h, const lfasm_mesh *m, int idx)
          return h->fail(LFASM_E_ARG, "mesh %d: attr[%d] = %d outside [1, %d]", idx, e, m->attr[e],
                         m->nreg);
       }
-      if (!(m->detJ[e] > 0.0))
+      if (geometry && !(m->detJ[e] > 0.0))
       {
          return h->fail(LFASM_E_ARG, "mesh %d: detJ[%d] <= 0 (expected positively oriented tet)",
                         idx, e);
@@ -1146,7 +1152,14 @@ void append(std::vector<T> &dst, const T *src, size_t n)
 
 extern "C" {
 
-int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
+}  // extern "C"
+
+namespace {
+
+// lfasm_init_batch / lfasm_init_vertices: xyz (single mesh only) = element
+// vertex coordinates, ne x 4 x 3, from which the geometry is computed on the
+// device instead of taken from the mesh's J, Jinv, detJ, v0.
+int init_impl(lfasm_handle *h, const lfasm_mesh *ms, int nmesh, const double *xyz)
 {
    if (!h) { return LFASM_E_STATE; }
    if (std::getenv("LITHOFEM_ASM_FORCE_FAIL"))
@@ -1159,7 +1172,7 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
    int64_t ne_tot = 0, n_tot = 0;
    for (int i = 0; i < nmesh; ++i)
    {
-      if ((rc = validate_mesh(h, &ms[i], i)) != LFASM_OK) { return rc; }
+      if ((rc = validate_mesh(h, &ms[i], i, xyz == nullptr)) != LFASM_OK) { return rc; }
       if (ms[i].p != ms[0].p || !same_rule(ms[i].rule_cc, ms[0].rule_cc, ms[0].nd) ||
           !same_rule(ms[i].rule_mass, ms[0].rule_mass, ms[0].nd) ||
           std::memcmp(ms[i].TInv, ms[0].TInv, 24 * sizeof(double)) != 0)
@@ -1204,12 +1217,48 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
    h->any_kpar = kpar;
    h->nreg = (int)(eps_tab.size() / 2);
    const size_t ne = (size_t)ne_tot;
+   if (nmesh == 1 && xyz)
+   {
+      // affine geometry from the vertex coordinates, on the device
+      double *dx = nullptr;
+      int *bad = nullptr;
+      int bad_first = INT32_MAX;
+      hipError_t e = upload(h, dx, xyz, ne * 12);
+      if (e == hipSuccess) { e = hipMalloc((void **)&h->J, ne * 9 * sizeof(double)); }
+      if (e == hipSuccess) { e = hipMalloc((void **)&h->Ji, ne * 9 * sizeof(double)); }
+      if (e == hipSuccess) { e = hipMalloc((void **)&h->detJ, ne * sizeof(double)); }
+      if (e == hipSuccess) { e = hipMalloc((void **)&h->v0, ne * 3 * sizeof(double)); }
+      if (e == hipSuccess) { e = upload(h, bad, &bad_first, 1); }
+      if (e == hipSuccess) { e = ev_begin(h, T_GEOM); }
+      if (e == hipSuccess)
+      {
+         hipLaunchKernelGGL(k_geometry, dim3((unsigned)std::min<size_t>((ne + 255) / 256, 8192)),
+                            dim3(256), 0, h->stream, (int)ne, (const double *)dx, h->J, h->Ji,
+                            h->detJ, h->v0, bad);
+         e = hipGetLastError();
+      }
+      if (e == hipSuccess) { e = ev_end(h, T_GEOM); }
+      if (e == hipSuccess) { e = hipMemcpyAsync(&bad_first, bad, sizeof(int), hipMemcpyDeviceToHost, h->stream); }
+      if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+      if (dx) { (void)hipFree(dx); }
+      if (bad) { (void)hipFree(bad); }
+      LF_HIP(e);
+      if (bad_first != INT32_MAX)
+      {
+         release_mesh(h);
+         return h->fail(LFASM_E_ARG, "element %d: detJ <= 0 (expected positively oriented tet)",
+                        bad_first);
+      }
+   }
    if (nmesh == 1)
    {
-      LF_HIP(upload(h, h->J, m0.J, ne * 9));
-      LF_HIP(upload(h, h->Ji, m0.Jinv, ne * 9));
-      LF_HIP(upload(h, h->detJ, m0.detJ, ne));
-      LF_HIP(upload(h, h->v0, m0.v0, ne * 3));
+      if (!xyz)
+      {
+         LF_HIP(upload(h, h->J, m0.J, ne * 9));
+         LF_HIP(upload(h, h->Ji, m0.Jinv, ne * 9));
+         LF_HIP(upload(h, h->detJ, m0.detJ, ne));
+         LF_HIP(upload(h, h->v0, m0.v0, ne * 3));
+      }
       LF_HIP(upload(h, h->attr, m0.attr, ne));
       LF_HIP(upload(h, h->dof, m0.dof, ne * nd));
       h->dof_host.assign(m0.dof, m0.dof + ne * nd);
@@ -1292,12 +1341,12 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
          const lfasm_mesh &m = ms[i];
          for (int e = 0; e < m.ne; ++e)
          {
-            const double z0 = m.v0[(size_t)e * 3 + 2];
-            const double *Je = m.J + (size_t)e * 9;
             bool inside = true;
             for (int v = 0; v < 4; ++v)
             {
-               const double z = v == 0 ? z0 : z0 + Je[6 + v - 1];
+               const double z = xyz ? xyz[(size_t)e * 12 + v * 3 + 2]
+                                    : (v == 0 ? m.v0[(size_t)e * 3 + 2]
+                                              : m.v0[(size_t)e * 3 + 2] + m.J[(size_t)e * 9 + 6 + v - 1]);
                inside = inside && z >= m.pml.z_min && z <= m.pml.z_max;
             }
             (inside ? cl : ql).push_back((int)(base + e));
@@ -1327,7 +1376,54 @@ int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
    return LFASM_OK;
 }
 
-int lfasm_init(lfasm_handle *h, const lfasm_mesh *m) { return lfasm_init_batch(h, m, 1); }
+}  // namespace
+
+extern "C" {
+
+int lfasm_init_batch(lfasm_handle *h, const lfasm_mesh *ms, int nmesh)
+{
+   return init_impl(h, ms, nmesh, nullptr);
+}
+
+int lfasm_init(lfasm_handle *h, const lfasm_mesh *m) { return init_impl(h, m, 1, nullptr); }
+
+int lfasm_init_vertices(lfasm_handle *h, const lfasm_mesh *m, const double *elem_xyz)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (!elem_xyz) { return h->fail(LFASM_E_ARG, "vertex coordinates missing"); }
+   return init_impl(h, m, 1, elem_xyz);
+}
+
+int lfasm_download_geometry(lfasm_handle *h, double *J, double *Jinv, double *detJ, double *v0)
+{
+   if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   const size_t ne = (size_t)h->ne;
+   if (J) { LF_HIP(hipMemcpyAsync(J, h->J, ne * 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream)); }
+   if (Jinv) { LF_HIP(hipMemcpyAsync(Jinv, h->Ji, ne * 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream)); }
+   if (detJ) { LF_HIP(hipMemcpyAsync(detJ, h->detJ, ne * sizeof(double), hipMemcpyDeviceToHost, h->stream)); }
+   if (v0) { LF_HIP(hipMemcpyAsync(v0, h->v0, ne * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream)); }
+   LF_HIP(hipStreamSynchronize(h->stream));
+   return LFASM_OK;
+}
+
+int lfasm_download_contributors(lfasm_handle *h, int *rc_ptr, int *rc_list)
+{
+   if (!h || !h->global_built) { return h ? h->fail(LFASM_E_STATE, "build_global first") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   if (rc_ptr)
+   {
+      LF_HIP(hipMemcpyAsync(rc_ptr, h->rc_ptr, ((size_t)h->height + 1) * sizeof(int),
+                            hipMemcpyDeviceToHost, h->stream));
+   }
+   if (rc_list)
+   {
+      LF_HIP(hipMemcpyAsync(rc_list, h->rc_list, (size_t)h->ne * h->nd * sizeof(int),
+                            hipMemcpyDeviceToHost, h->stream));
+   }
+   LF_HIP(hipStreamSynchronize(h->stream));
+   return LFASM_OK;
+}
 
 int lfasm_batch_offsets(const lfasm_handle *h, int64_t *elem_off, int64_t *row_off)
 {
@@ -1363,29 +1459,56 @@ int lfasm_set_eps_elem(lfasm_handle *h, const double *eps)
 
 namespace {
 
-// Per-row contributor lists (element, local index) in ascending element
-// order: counting sort of the element dof table.
-void host_contributors(int n, const std::vector<int> &dof, std::vector<int> &rc_ptr,
-                       std::vector<int> &rc_list)
+// Exclusive prefix sum of n int64 values into n + 1 (device, hipCUB);
+// returns the total through *total (host).
+hipError_t scan64(lfasm_handle *h, const int64_t *in, int64_t *out, int n, int64_t *total)
 {
-   rc_ptr.assign((size_t)n + 1, 0);
-   rc_list.assign(dof.size(), 0);
-   for (size_t i = 0; i < dof.size(); ++i) { rc_ptr[(size_t)dof[i] + 1]++; }
-   for (int r = 0; r < n; ++r) { rc_ptr[(size_t)r + 1] += rc_ptr[r]; }
-   std::vector<int> pos(rc_ptr.begin(), rc_ptr.end() - 1);
-   for (size_t i = 0; i < dof.size(); ++i) { rc_list[(size_t)pos[dof[i]]++] = (int)i; }
+   size_t tmp_bytes = 0;
+   hipError_t e = hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, in, out + 1, n, h->stream);
+   if (e != hipSuccess) { return e; }
+   void *tmp = nullptr;
+   if ((e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1))) != hipSuccess) { return e; }
+   e = hipMemsetAsync(out, 0, sizeof(int64_t), h->stream);
+   if (e == hipSuccess) { e = hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, in, out + 1, n, h->stream); }
+   if (e == hipSuccess)
+   {
+      e = hipMemcpyAsync(total, out + n, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream);
+   }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   (void)hipFree(tmp);
+   return e;
 }
 
-// Everything of build_global after the pattern (h->Ip, h->Jc on the device,
-// I on the host) and the contributor lists are in place.
-int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
-                  const std::vector<int> &rc_ptr, const std::vector<int> &rc_list,
-                  const unsigned char *is_ess)
+// The fused row units (tensor path mode 2) from host copies of the pattern
+// pieces they need; built only when that mode is selected.
+int ensure_fused_units(lfasm_handle *h)
+{
+   if (h->fused_tried || !h->global_built || !h->coef || h->p > 3) { return LFASM_OK; }
+   h->fused_tried = true;
+   const int n = h->height;
+   const size_t nk = (size_t)h->ne * h->nd;
+   std::vector<int> rc_ptr((size_t)n + 1), rc_list(nk);
+   std::vector<double> sign(nk);
+   std::vector<unsigned char> ess((size_t)n);
+   LF_HIP(hipMemcpy(rc_ptr.data(), h->rc_ptr, rc_ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
+   LF_HIP(hipMemcpy(rc_list.data(), h->rc_list, nk * sizeof(int), hipMemcpyDeviceToHost));
+   LF_HIP(hipMemcpy(sign.data(), h->sign, nk * sizeof(double), hipMemcpyDeviceToHost));
+   LF_HIP(hipMemcpy(ess.data(), h->ess, ess.size(), hipMemcpyDeviceToHost));
+   return build_fused_units(h, h->I_host.data(), h->dof_host, rc_ptr, rc_list, sign, ess);
+}
+
+// Everything of build_global after the pattern (h->Ip, h->Jc, h->rc_ptr,
+// h->rc_list on the device, h->I_host on the host): row-length limits,
+// essential mask, sign / essential flags and masks, slot map, the per-row
+// byte slot blocks -- all on the device.
+int finish_global(lfasm_handle *h, const unsigned char *is_ess)
 {
    const int n = h->height;
    const size_t ne = (size_t)h->ne, nd = (size_t)h->nd;
+   const int64_t nk = (int64_t)(ne * nd);
+   const std::vector<int> &I = h->I_host;
    int maxrow = 0;
-   for (int r = 0; r < n; ++r) { maxrow = std::max(maxrow, I[r + 1] - I[r]); }
+   for (int r = 0; r < n; ++r) { maxrow = std::max(maxrow, I[(size_t)r + 1] - I[(size_t)r]); }
    if (maxrow > 65535)
    {
       release_global(h);
@@ -1401,30 +1524,24 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
                      maxrow, lds_max / (int)(4 * sizeof(double2)));
    }
    h->maxrow = maxrow;
-   h->I_host.assign(I, I + (size_t)n + 1);
-   std::vector<unsigned char> ess((size_t)n, 0);
-   if (is_ess) { std::memcpy(ess.data(), is_ess, (size_t)n); }
-   LF_HIP(upload(h, h->ess, ess.data(), ess.size()));
-   // per (element, local dof) flags: bit0 sign < 0, bit1 essential dof
-   std::vector<double> sign(ne * nd);
-   LF_HIP(hipMemcpy(sign.data(), h->sign, sign.size() * sizeof(double), hipMemcpyDeviceToHost));
-   std::vector<unsigned char> eflag(ne * nd);
-   for (size_t i = 0; i < ne * nd; ++i)
+   if (is_ess) { LF_HIP(upload(h, h->ess, is_ess, (size_t)n)); }
+   else
    {
-      eflag[i] = (unsigned char)((sign[i] < 0.0 ? 1 : 0) | (ess[(size_t)dof[i]] ? 2 : 0));
+      LF_HIP(hipMalloc((void **)&h->ess, (size_t)std::max(n, 1)));
+      LF_HIP(hipMemsetAsync(h->ess, 0, (size_t)n, h->stream));
    }
-   LF_HIP(upload(h, h->eflag, eflag.data(), eflag.size()));
+   const unsigned grid = (unsigned)std::min<int64_t>((nk + 255) / 256, 8192);
+   // per (element, local dof) flags: bit0 sign < 0, bit1 essential dof
+   LF_HIP(hipMalloc((void **)&h->eflag, (size_t)std::max<int64_t>(nk, 1)));
+   hipLaunchKernelGGL(k_eflag, dim3(grid), dim3(256), 0, h->stream, nk, h->sign, h->dof, h->ess,
+                      h->eflag);
+   LF_HIP(hipGetLastError());
    if (nd <= 64)
    {
-      std::vector<uint64_t> mask(2 * ne, 0);
-      for (size_t e = 0; e < ne; ++e)
-         for (size_t b = 0; b < nd; ++b)
-         {
-            const unsigned char f = eflag[e * nd + b];
-            if (f & 1) { mask[2 * e] |= 1ull << b; }
-            if (f & 2) { mask[2 * e + 1] |= 1ull << b; }
-         }
-      LF_HIP(upload(h, h->emask, mask.data(), mask.size()));
+      LF_HIP(hipMalloc((void **)&h->emask, 2 * ne * sizeof(uint64_t)));
+      hipLaunchKernelGGL(k_emask, dim3((unsigned)std::min<size_t>((ne + 3) / 4, 8192)), dim3(256),
+                         0, h->stream, h->ne, h->nd, h->eflag, h->emask);
+      LF_HIP(hipGetLastError());
    }
    LF_HIP(hipMalloc((void **)&h->vals, (size_t)h->nnz * sizeof(double2)));
    // rows unpadded: padding them to aligned 128-byte lines measured no
@@ -1451,38 +1568,110 @@ int finish_global(lfasm_handle *h, const int *I, const std::vector<int> &dof,
       // byte slot rows regrouped per CSR row, in the row's contributor order
       // (every row has at most 256 entries): the scatter reads a row's slot
       // rows from one contiguous block
-      std::vector<int> rsoff((size_t)n + 1, 0);
-      for (int r = 0; r < n; ++r)
+      int64_t *len = nullptr, *off = nullptr;
+      int64_t total = 0;
+      hipError_t e = hipMalloc((void **)&len, (size_t)n * sizeof(int64_t));
+      if (e == hipSuccess) { e = hipMalloc((void **)&off, ((size_t)n + 1) * sizeof(int64_t)); }
+      if (e == hipSuccess)
       {
-         const int64_t len = (int64_t)(rc_ptr[(size_t)r + 1] - rc_ptr[(size_t)r]) * (int64_t)nd;
-         const int64_t next = rsoff[(size_t)r] + len;
-         if (next > INT32_MAX)
-         {
-            release_global(h);
-            return h->fail(LFASM_E_UNSUPPORTED, "slot blocks exceed 2^31 entries");
-         }
-         rsoff[(size_t)r + 1] = (int)next;
+         hipLaunchKernelGGL(k_slot_block_len, dim3((unsigned)std::min(8192, (n + 255) / 256)),
+                            dim3(256), 0, h->stream, n, h->nd, h->rc_ptr, len);
+         e = hipGetLastError();
       }
-      LF_HIP(upload(h, h->rsoff, rsoff.data(), rsoff.size()));
-      LF_HIP(hipMalloc((void **)&h->rslot, std::max<size_t>(rsoff[(size_t)n], 1)));
+      if (e == hipSuccess) { e = scan64(h, len, off, n, &total); }
+      if (e == hipSuccess && total <= INT32_MAX)
+      {
+         e = hipMalloc((void **)&h->rsoff, ((size_t)n + 1) * sizeof(int));
+         if (e == hipSuccess)
+         {
+            hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::min(8192, (n + 256) / 256)), dim3(256),
+                               0, h->stream, (int64_t)n + 1, (const int64_t *)off, h->rsoff);
+            e = hipGetLastError();
+         }
+         if (e == hipSuccess) { e = hipMalloc((void **)&h->rslot, std::max<size_t>((size_t)total, 1)); }
+      }
+      if (len) { (void)hipFree(len); }
+      if (off) { (void)hipFree(off); }
+      LF_HIP(e);
+      if (total > INT32_MAX)
+      {
+         release_global(h);
+         return h->fail(LFASM_E_UNSUPPORTED, "slot blocks exceed 2^31 entries");
+      }
       hipLaunchKernelGGL(k_row_slots, dim3((unsigned)((n + 3) / 4)), dim3(BLOCK), 0, h->stream, n,
                          (int)nd, h->sld, (const int *)h->rc_ptr, (const int *)h->rc_list,
                          (const uint16_t *)h->slot, (const int *)h->rsoff, h->rslot);
       LF_HIP(hipGetLastError());
    }
-   if (h->coef && h->p <= 3)
+   h->global_built = true;
+   if (h->tensor_mode == 2)
    {
-      const int rc = build_fused_units(h, I, dof, rc_ptr, rc_list, sign, ess);
+      const int rc = ensure_fused_units(h);
       if (rc != LFASM_OK) { release_global(h); return rc; }
    }
-   h->global_built = true;
    return LFASM_OK;
 }
 
-std::vector<int> host_dof(lfasm_handle *h, hipError_t &e)
+// Per-row contributor lists (ascending element, then local index) of the
+// element dof table on the device: histogram, scan, scattered fill, and a
+// sort of each row's segment.  Returns the largest row segment.
+int device_contributors(lfasm_handle *h, int *maxc_out)
 {
-   e = hipSuccess;
-   return h->dof_host;
+   const int n = h->height;
+   const int64_t nk = (int64_t)h->ne * h->nd;
+   const unsigned grid = (unsigned)std::min<int64_t>((nk + 255) / 256, 8192);
+   const unsigned rgrid = (unsigned)std::min(8192, (n + 255) / 256);
+   int *cnt = nullptr, *maxc = nullptr;
+   int64_t *cnt64 = nullptr, *ptr64 = nullptr;
+   int64_t total = 0;
+   hipError_t e = hipMalloc((void **)&cnt, ((size_t)n + 1) * sizeof(int));
+   if (e == hipSuccess) { e = hipMalloc((void **)&maxc, sizeof(int)); }
+   if (e == hipSuccess) { e = hipMalloc((void **)&cnt64, (size_t)n * sizeof(int64_t)); }
+   if (e == hipSuccess) { e = hipMalloc((void **)&ptr64, ((size_t)n + 1) * sizeof(int64_t)); }
+   if (e == hipSuccess) { e = hipMalloc((void **)&h->rc_ptr, ((size_t)n + 1) * sizeof(int)); }
+   if (e == hipSuccess) { e = hipMalloc((void **)&h->rc_list, (size_t)std::max<int64_t>(nk, 1) * sizeof(int)); }
+   if (e == hipSuccess) { e = hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int), h->stream); }
+   if (e == hipSuccess) { e = hipMemsetAsync(maxc, 0, sizeof(int), h->stream); }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_contrib_count, dim3(grid), dim3(256), 0, h->stream, nk, h->dof, cnt);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess)
+   {
+      // the counts widened to 64 bits for the scan (nk fits in int32, checked at init)
+      hipLaunchKernelGGL(k_widen, dim3(rgrid), dim3(256), 0, h->stream, n, (const int *)cnt, cnt64);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess) { e = scan64(h, cnt64, ptr64, n, &total); }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::min(8192, (n + 256) / 256)), dim3(256), 0,
+                         h->stream, (int64_t)n + 1, (const int64_t *)ptr64, h->rc_ptr);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess) { e = hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int), h->stream); }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_contrib_fill, dim3(grid), dim3(256), 0, h->stream, nk, h->dof,
+                         (const int *)h->rc_ptr, cnt, h->rc_list);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_contrib_sort, dim3(rgrid), dim3(256), 0, h->stream, n,
+                         (const int *)h->rc_ptr, h->rc_list, maxc);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess) { e = hipMemcpyAsync(maxc_out, maxc, sizeof(int), hipMemcpyDeviceToHost, h->stream); }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   for (void *q : {(void *)cnt, (void *)maxc, (void *)cnt64, (void *)ptr64})
+   {
+      if (q) { (void)hipFree(q); }
+   }
+   LF_HIP(e);
+   if (total != nk) { return h->fail(LFASM_E_STATE, "contributor count %lld != %lld", (long long)total, (long long)nk); }
+   return LFASM_OK;
 }
 
 }  // namespace
@@ -1510,16 +1699,13 @@ int lfasm_build_global(lfasm_handle *h, const int *I, const int *Jcol, int64_t n
    LF_HIP(hipSetDevice(h->device));
    release_global(h);
    h->nnz = nnz;
-   hipError_t e = hipSuccess;
-   const std::vector<int> dof = host_dof(h, e);
-   LF_HIP(e);
-   std::vector<int> rc_ptr, rc_list;
-   host_contributors(n, dof, rc_ptr, rc_list);
+   h->I_host.assign(I, I + (size_t)n + 1);
    LF_HIP(upload(h, h->Ip, I, (size_t)n + 1));
    LF_HIP(upload(h, h->Jc, Jcol, (size_t)nnz));
-   LF_HIP(upload(h, h->rc_ptr, rc_ptr.data(), rc_ptr.size()));
-   LF_HIP(upload(h, h->rc_list, rc_list.data(), rc_list.size()));
-   return finish_global(h, I, dof, rc_ptr, rc_list, is_ess);
+   int maxc = 0;
+   const int rc = device_contributors(h, &maxc);
+   if (rc != LFASM_OK) { release_global(h); return rc; }
+   return finish_global(h, is_ess);
 }
 
 int lfasm_build_global_device(lfasm_handle *h, const unsigned char *is_ess)
@@ -1528,52 +1714,71 @@ int lfasm_build_global_device(lfasm_handle *h, const unsigned char *is_ess)
    LF_HIP(hipSetDevice(h->device));
    release_global(h);
    const int n = h->height, nd = h->nd;
-   hipError_t e = hipSuccess;
-   const std::vector<int> dof = host_dof(h, e);
-   LF_HIP(e);
-   std::vector<int> rc_ptr, rc_list;
-   host_contributors(n, dof, rc_ptr, rc_list);
    int maxc = 0;
-   for (int r = 0; r < n; ++r) { maxc = std::max(maxc, rc_ptr[(size_t)r + 1] - rc_ptr[r]); }
+   int rc = device_contributors(h, &maxc);
+   if (rc != LFASM_OK) { release_global(h); return rc; }
    int cap = 64;
    while (cap < maxc * nd) { cap <<= 1; }
    const size_t shmem = (size_t)PAT_WAVES * cap * sizeof(int);
    if (shmem > h->lds_max)
    {
+      release_global(h);
       return h->fail(LFASM_E_UNSUPPORTED, "%d contributors x %d dofs per row exceed the LDS sort",
                      maxc, nd);
    }
-   LF_HIP(upload(h, h->rc_ptr, rc_ptr.data(), rc_ptr.size()));
-   LF_HIP(upload(h, h->rc_list, rc_list.data(), rc_list.size()));
    LF_HIP(hipMalloc((void **)&h->Ip, ((size_t)n + 1) * sizeof(int)));
    LF_HIP(hipFuncSetAttribute((const void *)k_csr_rows,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
    const dim3 grid((unsigned)((n + PAT_WAVES - 1) / PAT_WAVES));
-   LF_HIP(ev_begin(h, T_PATTERN));
-   hipLaunchKernelGGL(k_csr_rows, grid, dim3(64 * PAT_WAVES), shmem, h->stream, n, nd, h->dof,
-                      h->rc_ptr, h->rc_list, cap, 0, h->Ip, (int *)nullptr);
-   LF_HIP(hipGetLastError());
-   // row pointers: exclusive scan of the counts on the host (n + 1 ints)
-   std::vector<int> I((size_t)n + 1, 0);
-   LF_HIP(hipMemcpyAsync(I.data() + 1, h->Ip, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
-                         h->stream));
-   LF_HIP(hipStreamSynchronize(h->stream));
+   int *cnt = nullptr;
+   int64_t *cnt64 = nullptr, *ptr64 = nullptr;
    int64_t nnz = 0;
-   for (int r = 0; r < n; ++r)
+   hipError_t e = hipMalloc((void **)&cnt, (size_t)n * sizeof(int));
+   if (e == hipSuccess) { e = hipMalloc((void **)&cnt64, (size_t)n * sizeof(int64_t)); }
+   if (e == hipSuccess) { e = hipMalloc((void **)&ptr64, ((size_t)n + 1) * sizeof(int64_t)); }
+   if (e == hipSuccess) { e = ev_begin(h, T_PATTERN); }
+   if (e == hipSuccess)
    {
-      nnz += I[(size_t)r + 1];
-      if (nnz > INT32_MAX) { release_global(h); return h->fail(LFASM_E_UNSUPPORTED, "nnz > int32"); }
-      I[(size_t)r + 1] = (int)nnz;
+      hipLaunchKernelGGL(k_csr_rows, grid, dim3(64 * PAT_WAVES), shmem, h->stream, n, nd, h->dof,
+                         h->rc_ptr, h->rc_list, cap, 0, cnt, (int *)nullptr);
+      e = hipGetLastError();
    }
+   if (e == hipSuccess)
+   {
+      hipLaunchKernelGGL(k_widen, dim3((unsigned)std::min(8192, (n + 255) / 256)), dim3(256), 0,
+                         h->stream, n, (const int *)cnt, cnt64);
+      e = hipGetLastError();
+   }
+   // row pointers: a device scan of the counts (64-bit, checked against int32)
+   if (e == hipSuccess) { e = scan64(h, cnt64, ptr64, n, &nnz); }
+   if (e == hipSuccess && nnz <= INT32_MAX)
+   {
+      hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::min(8192, (n + 256) / 256)), dim3(256), 0,
+                         h->stream, (int64_t)n + 1, (const int64_t *)ptr64, h->Ip);
+      e = hipGetLastError();
+      if (e == hipSuccess) { e = hipMalloc((void **)&h->Jc, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)); }
+      if (e == hipSuccess)
+      {
+         hipLaunchKernelGGL(k_csr_rows, grid, dim3(64 * PAT_WAVES), shmem, h->stream, n, nd, h->dof,
+                            h->rc_ptr, h->rc_list, cap, 1, h->Ip, h->Jc);
+         e = hipGetLastError();
+      }
+      if (e == hipSuccess) { e = ev_end(h, T_PATTERN); }
+   }
+   for (void *q : {(void *)cnt, (void *)cnt64, (void *)ptr64})
+   {
+      if (q) { (void)hipFree(q); }
+   }
+   if (e != hipSuccess) { release_global(h); }
+   LF_HIP(e);
+   if (nnz > INT32_MAX) { release_global(h); return h->fail(LFASM_E_UNSUPPORTED, "nnz > int32"); }
    h->nnz = nnz;
-   LF_HIP(hipMemcpyAsync(h->Ip, I.data(), I.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-   LF_HIP(hipMalloc((void **)&h->Jc, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int)));
-   hipLaunchKernelGGL(k_csr_rows, grid, dim3(64 * PAT_WAVES), shmem, h->stream, n, nd, h->dof,
-                      h->rc_ptr, h->rc_list, cap, 1, h->Ip, h->Jc);
-   LF_HIP(hipGetLastError());
-   LF_HIP(ev_end(h, T_PATTERN));
+   // the host copy of the row pointers (n + 1 ints) kept for the variants
+   h->I_host.resize((size_t)n + 1);
+   LF_HIP(hipMemcpyAsync(h->I_host.data(), h->Ip, ((size_t)n + 1) * sizeof(int),
+                         hipMemcpyDeviceToHost, h->stream));
    LF_HIP(hipStreamSynchronize(h->stream));
-   return finish_global(h, I.data(), dof, rc_ptr, rc_list, is_ess);
+   return finish_global(h, is_ess);
 }
 
 int lfasm_download_pattern(lfasm_handle *h, int *I, int *Jcol)
@@ -1628,6 +1833,11 @@ int lfasm_set_tensor_path(lfasm_handle *h, int mode)
    if (mode < 0 || mode > 2) { return h->fail(LFASM_E_ARG, "tensor path mode %d outside 0..2", mode); }
    h->tensor_mode = mode;
    drop_graph(h);
+   if (mode == 2)
+   {
+      const int rc = ensure_fused_units(h);
+      if (rc != LFASM_OK) { return rc; }
+   }
    return mode ? h->nconst : 0;
 }
 

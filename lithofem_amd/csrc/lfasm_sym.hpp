@@ -246,7 +246,10 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, dbl4 (&t)[2],
 // before the MFMAs of chunk c, its rows formed after them); NBUF = 1: one
 // staging buffer (half the LDS, more workgroups per CU), the next chunk
 // formed between two barriers.
-template <int NBUF, int WAVES_PER_EU>
+// PRIO: the wavefronts raise their issue priority while they run the MFMAs
+// of a chunk and drop it for the staging, so that on a SIMD shared by
+// several elements the matrix pipe is fed first.
+template <int NBUF, int WAVES_PER_EU, bool PRIO = false>
 __global__ void __launch_bounds__(SYM_THREADS)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
 k_local_mfma_sym(LocalParams P, int nqt)
@@ -291,6 +294,7 @@ k_local_mfma_sym(LocalParams P, int nqt)
       const bool more = ch + 1 < nch;
       const SymItem nxt = sym_load_item(P, ss, more ? ch + 1 : ch, tid);
       const bool cross = ss.cross && ch >= ss.nch_cc;   // wave-uniform
+      if (PRIO) { __builtin_amdgcn_s_setprio(2); }
 #pragma unroll
       for (int ks = 0; ks < SYM_ROWS / 4; ++ks)
       {
@@ -315,6 +319,7 @@ k_local_mfma_sym(LocalParams P, int nqt)
             pdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdq, pdd[1], 0, 0, 0);
          }
       }
+      if (PRIO) { __builtin_amdgcn_s_setprio(0); }
       if (NBUF == 1) { __syncthreads(); }
       if (more) { sym_form_item(P, ec, ss, qcoef, nxt, buf + (NBUF == 2 ? ((ch + 1) & 1) * SYM_STG : 0)); }
       __syncthreads();

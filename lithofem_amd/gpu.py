@@ -124,6 +124,9 @@ def lib():
         "lfasm_step_capture": ([vp, i, i, i, i], i),
         "lfasm_step_launch": ([vp, i], i),
         "lfasm_build_global_device": ([vp, vp], i),
+        "lfasm_init_vertices": ([vp, vp, vp], i),
+        "lfasm_download_geometry": ([vp, vp, vp, vp, vp], i),
+        "lfasm_download_contributors": ([vp, vp, vp], i),
         "lfasm_download_pattern": ([vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
@@ -223,6 +226,32 @@ class AsmGpuGlobal:
         self._check(lib().lfasm_init(self._h, ct.byref(m)))
         self.ne, self.nd, self.n = ad.ne, ad.nd, ad.height
         self.nmesh = 1
+
+    def init_vertices(self, ad, problem, elem_xyz, eps_elem=None):
+        """AsmGpu::Init with the geometry computed on the device from the
+        element vertex coordinates (ne, 4, 3); ad's J, Jinv, detJ, v0 are not
+        read."""
+        keep = []
+        m = self._mesh_struct(ad, problem, eps_elem, keep)
+        x = np.ascontiguousarray(elem_xyz, dtype=np.float64)
+        assert x.shape == (ad.ne, 4, 3)
+        self._check(lib().lfasm_init_vertices(self._h, ct.byref(m), _p(x)))
+        self.ne, self.nd, self.n = ad.ne, ad.nd, ad.height
+        self.nmesh = 1
+
+    def download_geometry(self):
+        J = np.empty((self.ne, 9))
+        Ji = np.empty((self.ne, 9))
+        det = np.empty(self.ne)
+        v0 = np.empty((self.ne, 3))
+        self._check(lib().lfasm_download_geometry(self._h, _p(J), _p(Ji), _p(det), _p(v0)))
+        return J, Ji, det, v0
+
+    def download_contributors(self):
+        ptr = np.empty(self.n + 1, dtype=np.int32)
+        lst = np.empty(self.ne * self.nd, dtype=np.int32)
+        self._check(lib().lfasm_download_contributors(self._h, _p(ptr), _p(lst)))
+        return ptr, lst
 
     def init_batch(self, items):
         """items: sequence of (AsmData, Problem[, eps_elem]); the meshes are
@@ -380,11 +409,11 @@ class AsmGpuGlobal:
         return out
 
     def timings(self) -> dict:
-        ms = np.zeros(9)
-        self._check(lib().lfasm_timings(self._h, _p(ms), 9))
+        ms = np.zeros(10)
+        self._check(lib().lfasm_timings(self._h, _p(ms), 10))
         return dict(local_ms=ms[0], scatter_ms=ms[1], build_ms=ms[2], batch_ms=ms[3],
                     adjoint_ms=ms[4], local_tensor_ms=ms[5], local_quad_ms=ms[6],
-                    step_ms=ms[7], pattern_ms=ms[8])
+                    step_ms=ms[7], pattern_ms=ms[8], geometry_ms=ms[9])
 
     # -- captured step (HIP graph)
     def step_capture(self, mask_cc=CURLCURL, mask_mass=MASS_ALL, apply_te=True, eliminate=True):

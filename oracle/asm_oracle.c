@@ -861,3 +861,36 @@ void orc_rhs_scale_all(const orc_asm_data *ad, const orc_problem *pb, const orc_
    }
    free(be);
 }
+
+/* Affine geometry of ne elements from their vertex coordinates (ne x 4 x 3),
+ * in the operation order of the library's device extraction: J columns
+ * x_{j+1} - x_0, detJ by cofactors of the first row, J^-1 = adj(J) / detJ,
+ * v0 = x_0 (the quantities solver/asm_data.inc:152-171 reads from the
+ * element transformation). */
+void orc_geometry(int ne, const double *xyz, double *J, double *Jinv, double *detJ, double *v0)
+{
+   for (int e = 0; e < ne; ++e)
+   {
+      const double *x = xyz + (size_t)e * 12;
+      double a[9];
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) { a[i * 3 + j] = x[(j + 1) * 3 + i] - x[i]; }
+      const double c00 = a[4] * a[8] - a[5] * a[7];
+      const double c01 = a[5] * a[6] - a[3] * a[8];
+      const double c02 = a[3] * a[7] - a[4] * a[6];
+      const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+      double *iv = Jinv + (size_t)e * 9;
+      iv[0] = c00 / det;
+      iv[3] = c01 / det;
+      iv[6] = c02 / det;
+      iv[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+      iv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+      iv[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+      iv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+      iv[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+      iv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+      for (int i = 0; i < 9; ++i) { J[(size_t)e * 9 + i] = a[i]; }
+      detJ[e] = det;
+      for (int d = 0; d < 3; ++d) { v0[(size_t)e * 3 + d] = x[d]; }
+   }
+}

@@ -160,6 +160,11 @@ int orc_scale_scatter_rows(const orc_asm_data *ad, const double *S_all, int r0, 
 void orc_rhs_scale_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incident *inc,
                        int rhs, double *s);
 
+/* Affine element geometry from vertex coordinates (ne x 4 x 3) in the
+ * operation order of the library's device extraction (pinned: J Jinv = I,
+ * 6 |vol| = detJ against the mesh producer, v0 = the first vertex). */
+void orc_geometry(int ne, const double *xyz, double *J, double *Jinv, double *detJ, double *v0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -339,6 +339,17 @@ def eliminate(I, Jcol, is_ess, vre, vim):
     return vre, vim
 
 
+def geometry(elem_xyz):
+    """J, Jinv (ne x 9), detJ, v0 of the elements' vertex coordinates."""
+    x = np.ascontiguousarray(elem_xyz, dtype=np.float64)
+    ne = x.shape[0]
+    J, Ji, det, v0 = np.empty((ne, 9)), np.empty((ne, 9)), np.empty(ne), np.empty((ne, 3))
+    L = lib()
+    L.orc_geometry.argtypes = [ct.c_int] + [ct.c_void_p] * 5
+    L.orc_geometry(ne, _ptr(x), _ptr(J), _ptr(Ji), _ptr(det), _ptr(v0))
+    return J, Ji, det, v0
+
+
 def eliminate_scale(I, Jcol, is_ess, s):
     """Scale of the eliminated system: essential rows and columns are set,
     not computed (0 off the diagonal, 1 on it)."""

@@ -452,6 +452,27 @@ def test_per_nonzero_metric_and_scales(name, p):
     assert np.all(np.abs(np.diag(Ec)) <= np.diag(Sc) * (1 + 1e-12))
 
 
+def test_geometry_from_vertices():
+    """orc_geometry (the operation order of the device extraction): J columns
+    are the element's edge vectors from vertex 0, J J^-1 = I, detJ = 6 x the
+    signed volume from the scalar triple product, v0 = vertex 0, and the
+    mesh producer's numpy geometry agrees to rounding."""
+    c, ad, I, J = case("hole", 1, 6.0, "small")
+    x = np.asarray(c.mesh.elem_xyz, dtype=np.float64)
+    Jo, Jio, det, v0 = O.geometry(x)
+    A = Jo.reshape(-1, 3, 3)
+    for j in range(3):
+        assert np.array_equal(A[:, :, j], x[:, j + 1] - x[:, 0])
+    assert np.array_equal(v0, x[:, 0])
+    eye = np.einsum("eij,ejk->eik", A, Jio.reshape(-1, 3, 3))
+    assert np.abs(eye - np.eye(3)).max() < 1e-13
+    e1, e2, e3 = x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]
+    triple = np.einsum("ei,ei->e", e1, np.cross(e2, e3))
+    assert np.abs(det - triple).max() <= 1e-13 * np.abs(triple).max()
+    assert np.abs(det - ad.detJ).max() <= 1e-13 * np.abs(ad.detJ).max()
+    assert np.abs(Jio - ad.Jinv).max() <= 1e-12 * np.abs(ad.Jinv).max()
+
+
 def test_threaded_oracle_equals_serial():
     """The host-thread forms used at full size (element blocks; disjoint CSR
     row ranges of the scatter) are bit-identical to the serial oracle."""
