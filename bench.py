@@ -132,13 +132,18 @@ def cpu_baseline_threads(wl, I, J) -> dict:
                         f"{t3 - t2:.2f} s"))
 
 
-def sample_parity(wl, I, J, Ag, e0: int, n: int = 192) -> dict:
+def sample_parity(wl, I, J, Ag, e0: int, xyz, n: int = 192) -> dict:
     """GPU element matrices of `n` elements of the baseline sample against the
-    parity build of the oracle: max-normalised error, error per entry
+    parity build of the oracle (on the geometry the device formed from the
+    vertex coordinates, orc_geometry): max-normalised error, error per entry
     against the entries' componentwise scales, and plain relative error per
     entry over the entries >= 1e-8 of the largest."""
+    import dataclasses
+
     from oracle import oracle as O
-    orc = O.Oracle(wl.ad, wl.cell.problem, I, J)
+    Jo, Jio, deto, v0o = O.geometry(xyz)
+    ado = dataclasses.replace(wl.ad, J=Jo, Jinv=Jio, detJ=deto, v0=v0o)
+    orc = O.Oracle(ado, wl.cell.problem, I, J)
     ref = orc.elements(e0, e0 + n)
     sc = orc.scale_elements(np.arange(e0, e0 + n))
     got = Ag[e0:e0 + n]
@@ -207,27 +212,37 @@ def main() -> int:
 
     import torch
     torch.cuda.init()
+    # host setup path: host geometry + host CSR symbolic + init + build_global
+    gh = gpu.AsmGpuGlobal(0)
+    t0 = time.perf_counter()
+    gh.init(ad, pb)
+    gh.build_global(I, J, ad.ess)
+    t_upload = time.perf_counter() - t0
+    gh.close()
+    del gh
+    # device setup path (the one the step runs on): geometry from the vertex
+    # coordinates, contributor lists and CSR pattern all on the device
+    xyz = np.ascontiguousarray(wl.cell.mesh.elem_xyz, dtype=np.float64)
     g = gpu.AsmGpuGlobal(0)
     t0 = time.perf_counter()
-    g.init(ad, pb)
-    g.build_global(I, J, ad.ess)
-    t_upload = time.perf_counter() - t0
-    # the same pattern built on the device (replaces the host symbolic phase)
+    g.init_vertices(ad, pb, xyz)
+    t_init_v = time.perf_counter() - t0
     t0 = time.perf_counter()
     g.build_global_device(ad.ess)
     t_dev_build = time.perf_counter() - t0
-    pattern_ms = g.timings()["pattern_ms"]
+    tim = g.timings()
+    pattern_ms, geometry_ms = tim["pattern_ms"], tim["geometry_ms"]
     Id, Jd = g.download_pattern()
     same_pattern = bool(np.array_equal(Id, I) and np.array_equal(Jd, J))
     del Id, Jd
-    log(f"device pattern: build_global_device {t_dev_build:.3f}s (pattern phase "
-        f"{pattern_ms:.1f} ms) vs host csr_symbolic {t_csr:.3f}s + build_global {t_upload:.3f}s; "
-        f"identical: {same_pattern}")
+    log(f"device setup: init_vertices {t_init_v:.3f}s (geometry kernel {geometry_ms:.2f} ms) + "
+        f"build_global_device {t_dev_build:.3f}s (pattern phase {pattern_ms:.1f} ms) vs host "
+        f"extract {t_extract:.2f}s + csr_symbolic {t_csr:.3f}s + init+build_global "
+        f"{t_upload:.3f}s; pattern identical: {same_pattern}")
     nconst = g.set_tensor_path({"on": 1, "off": 0, "fused": 2}[args.tensor])
     nquad = ad.ne - nconst
     log(f"element classes: {nconst} constant-coefficient (reference-tensor kernel), "
         f"{nquad} PML (quadrature kernel)")
-    log(f"upload+build_global {t_upload:.2f}s (device part {g.timings()['build_ms']:.2f} ms)")
     tuned = None
     if args.kernel == 0:
         from lithofem_amd.tune import tune_local
@@ -364,9 +379,12 @@ def main() -> int:
                    "download_values_ms": download_ms,
                    "elements_per_s_incl_download": ad.ne / ((ms_step + download_ms) * 1e-3),
                    "local_min_bytes": local_bytes(ad.ne, ad.nd),
-                   "setup_s": {"extract": t_extract, "csr_host_threads": t_csr,
-                               "init_and_build_global": t_upload,
-                               "build_global_device": t_dev_build,
+                   "setup_s": {"host_path": {"extract_incl_geometry": t_extract,
+                                             "csr_host_threads": t_csr,
+                                             "init_and_build_global": t_upload},
+                               "device_path": {"init_vertices": t_init_v,
+                                               "geometry_kernel_ms": geometry_ms,
+                                               "build_global_device": t_dev_build},
                                "device_pattern_phase_ms": pattern_ms,
                                "device_pattern_identical": same_pattern}},
     }
@@ -374,7 +392,7 @@ def main() -> int:
         cb = cpu_baseline(wl, I, J)
         with _stdout_to_stderr():
             Ag = g.download_local()
-        par = sample_parity(wl, I, J, Ag, cb["_e0"])
+        par = sample_parity(wl, I, J, Ag, cb["_e0"], xyz)
         del Ag
         log(f"cpu baseline {cb['value']:.1f} elements/s ({cb['sample']}); sample parity {par}")
         out["cpu_baseline"] = {k: v for k, v in cb.items() if not k.startswith("_")}
