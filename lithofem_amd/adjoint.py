@@ -150,7 +150,7 @@ class EpsForwardModel:
     """GPU assembly + host solve as a function of per-element eps."""
 
     def __init__(self, ad: AsmData, problem, I, J, params, incident=None, rhs_col=0,
-                 device=0):
+                 device=0, refine=1):
         self.ad, self.problem = ad, problem
         self.I, self.J = I, J
         self.params = np.asarray(params, dtype=np.int32)
@@ -161,6 +161,7 @@ class EpsForwardModel:
         self.g.build_global(I, J, ad.ess)
         if incident is not None:
             self.g.set_incident(incident)
+        self.refine = refine         # iterative-refinement steps of the host solves
         self.ess_mask = np.zeros(ad.height, bool)
         self.ess_mask[ad.ess] = True
         self.lu = None
@@ -184,7 +185,7 @@ class EpsForwardModel:
         else:
             b = np.zeros(self.ad.height, dtype=np.complex128)
         t.append(time.perf_counter())
-        self.lu = HostLU(self.I, self.J, vre + 1j * vim, self.ad.height)
+        self.lu = HostLU(self.I, self.J, vre + 1j * vim, self.ad.height, refine=self.refine)
         t.append(time.perf_counter())
         self.u = self.lu.solve(b)
         t.append(time.perf_counter())

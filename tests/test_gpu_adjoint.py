@@ -6,7 +6,10 @@ GPU matrix + RHS assembly -> host LU solve.  Objective: reflected +-1
 order efficiencies + mean squared deviation of |E|^2 on the z = -1.6 nm
 slice from a seeded random target.  The gradient comes from
 torch.autograd through EpsToField (transpose solve + eps-sensitivity
-kernel); central differences use a relative step of 1e-6.
+kernel); central differences use a relative step of 1e-6 on 32 random
+elements in the real and the imaginary direction, each within 1e-6 of the
+adjoint gradient (the solves of the difference quotients take 3 steps of
+iterative refinement, so their rounding stays far below the step).
 """
 
 from __future__ import annotations
@@ -30,7 +33,7 @@ def _setup(p=2):
     I, J = csr_symbolic(ad.dof, ad.height)
     inc = incident_set([pb], [[(6.0, 0.0, "s")]])
     params = np.nonzero((ad.attr == 2) | (ad.attr == 3))[0].astype(np.int32)
-    model = EpsForwardModel(ad, pb, I, J, params, incident=inc, rhs_col=0)
+    model = EpsForwardModel(ad, pb, I, J, params, incident=inc, rhs_col=0, refine=3)
     rng = np.random.default_rng(3)
     ng = 8
     target = rng.uniform(0, 1, (ng, ng))
@@ -59,7 +62,8 @@ def test_adjoint_gradient_matches_central_differences():
         print(f"J = {float(J.detach()):.6e}; nparam = {len(params)}; |grad| = {np.abs(grad).max():.3e}",
               flush=True)
         rng = np.random.default_rng(4)
-        picks = rng.choice(len(params), 6, replace=False)
+        picks = rng.choice(len(params), 32, replace=False)
+        worst = 0.0
         for i in picks:
             for dirn in (1.0, 1j):
                 h = 1e-6 * abs(eps0[i])
@@ -73,7 +77,10 @@ def test_adjoint_gradient_matches_central_differences():
                 rel = abs(fd - an) / max(abs(an), 1e-30)
                 print(f"  elem {params[i]} dir {dirn}: adjoint {an:.9e} fd {fd:.9e} rel {rel:.2e}",
                       flush=True)
-                assert rel < 1e-5, (i, dirn, an, fd)
+                worst = max(worst, rel)
+        print(f"worst relative difference over {len(picks)} elements x 2 directions: {worst:.2e}",
+              flush=True)
+        assert worst < 1e-6, worst
     finally:
         model.close()
 
@@ -101,6 +108,6 @@ def test_region_sum_matches_two_solve_difference():
                 rel = abs(fd - an) / max(abs(an), 1e-30)
                 print(f"  region {attr} dir {dirn}: sum g_e {an:.9e} two-solve {fd:.9e} rel {rel:.2e}",
                       flush=True)
-                assert rel < 1e-5, (attr, dirn, an, fd)
+                assert rel < 1e-6, (attr, dirn, an, fd)
     finally:
         model.close()
