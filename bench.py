@@ -426,7 +426,7 @@ def measure_peaks(device: int = 0) -> dict:
     L = ct.CDLL(str(ROOT / "lithofem_amd" / "libpeaks.so"))
     L.lfpeaks_measure.argtypes = [ct.c_int, ct.c_void_p, ct.c_int]
     L.lfpeaks_measure.restype = ct.c_int
-    out = np.zeros(11 + 4 * 16)
+    out = np.zeros(11 + 4 * 16 + 3)
     rows = L.lfpeaks_measure(device, out.ctypes.data, len(out))
     if rows < 0:
         raise RuntimeError(f"lfpeaks_measure failed ({rows})")
@@ -439,7 +439,9 @@ def measure_peaks(device: int = 0) -> dict:
             "fp64_valu_tflops": out[4], "fp64_valu_clock_ghz": out[5],
             "fp64_valu_at_clock_peak_tflops": cus * 4 * 32 * out[5] * 1e9 / 1e12,
             "hbm_copy_gbps": out[6], "hbm_write_gbps": out[7], "hbm_read_gbps": out[8],
-            "cus": cus, "reported_max_clock_ghz": out[10], "mfma_sweep": sweep}
+            "cus": cus, "reported_max_clock_ghz": out[10], "mfma_sweep": sweep,
+            "mfma_valu_overlap_ms": {"mfma_alone": out[75], "valu_alone": out[76],
+                                     "interleaved": out[77]}}
 
 
 def _quad_kernel_name(selector: int, nd: int) -> str:
@@ -448,9 +450,9 @@ def _quad_kernel_name(selector: int, nd: int) -> str:
     if not 32 < nd <= 48:
         return f"selector {selector}"
     return {1: "k_local_generic<4, 512> + k_te", 2: "k_local_mfma_sym<1, 3>",
-            3: "k_local_mfma_pipe<3, 2, 2>", 4: "k_local_mfma_pipe<3, 1, 4>",
+            3: "k_local_mfma_sym<1, 4>", 4: "k_local_mfma_pipe<3, 1, 4>",
             5: "k_local_mfma<3, 1, 1, 2>", 6: "k_local_mfma_sym<2, 3>",
-            7: "k_local_mfma_pipe<3, 1, 2>"}.get(selector, f"selector {selector}")
+            7: "k_local_mfma_sym<1, 3, true>"}.get(selector, f"selector {selector}")
 
 
 def _measured_traffic() -> dict:

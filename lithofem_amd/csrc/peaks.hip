@@ -74,6 +74,36 @@ __global__ void k_peak_valu(int iters, double y, double *__restrict__ sink,
    if (s == -1.0) { sink[blockIdx.x] = s; }
 }
 
+// Does FP64 VALU work overlap FP64 MFMA work?  Each wave runs `iters`
+// rounds of NM MFMAs on 4 accumulator chains followed by NV independent FMAs
+// on 8 chains; with the two kinds overlapping, the time of the mix is the
+// larger of the two parts, with a shared pipe it is their sum.
+template <int NM, int NV>
+__global__ void __launch_bounds__(256) k_peak_mix(int iters, double y, double *__restrict__ sink)
+{
+   const int lane = threadIdx.x & 63;
+   const double a = 1.0 + 1e-9 * lane, b = 1.0 - 1e-9 * lane;
+   pdbl4 acc[4];
+#pragma unroll
+   for (int i = 0; i < 4; ++i) { acc[i] = pdbl4{0.0, 0.0, 0.0, 0.0}; }
+   double x[8];
+#pragma unroll
+   for (int i = 0; i < 8; ++i) { x[i] = 1.0 + i * 1e-3 + threadIdx.x * 1e-9; }
+   for (int it = 0; it < iters; ++it)
+   {
+#pragma unroll
+      for (int m = 0; m < NM; ++m) { acc[m & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[m & 3], 0, 0, 0); }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) { x[v & 7] = __fma_rn(x[v & 7], y, 1e-12); }
+   }
+   double s = 0.0;
+#pragma unroll
+   for (int i = 0; i < 4; ++i) { s += acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3]; }
+#pragma unroll
+   for (int i = 0; i < 8; ++i) { s += x[i]; }
+   if (s == -1.0) { sink[blockIdx.x] = s; }
+}
+
 __global__ void k_fill_zero(double2 *p, size_t n)
 {
    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -227,19 +257,21 @@ void mfma_case(Ctx &x, int ncu, int bpc, double *sink, long long *clk, double *r
 
 extern "C" {
 
-/* out (nout >= 11 + 4 * 12 doubles):
+/* out (nout >= 11 + 4 * 16 + 3 doubles):
  *  [0] best FP64 MFMA TFLOP/s, [1] shader clock during it (GHz),
  *  [2] its chains per wave, [3] its waves per SIMD,
  *  [4] FP64 VALU FMA TFLOP/s, [5] shader clock during it (GHz),
  *  [6] HBM copy GB/s (read + write bytes), [7] write-only GB/s, [8] read-only GB/s,
  *  [9] compute units, [10] reported maximum clock (GHz),
- *  [11 + 4 i ...] the MFMA sweep: chains, waves per SIMD, TFLOP/s, clock GHz.
+ *  [11 + 4 i ...] the MFMA sweep: chains, waves per SIMD, TFLOP/s, clock GHz
+ *  (i < 16), then [11 + 64 ..] the overlap probe: ms of MFMAs alone, FMAs
+ *  alone, both interleaved.
  * Each stream figure is the best over launch shapes and ordinary /
  * non-temporal accesses of 2 GiB buffers.  Returns the number of sweep rows
  * or a negative HIP error. */
 int lfpeaks_measure(int device, double *out, int nout)
 {
-   if (!out || nout < 11 + 4 * 12) { return -1; }
+   if (!out || nout < 11 + 4 * 16 + 3) { return -1; }
    for (int i = 0; i < nout; ++i) { out[i] = 0.0; }
    if (hipSetDevice(device) != hipSuccess) { return -2; }
    int ncu = 0, wall_khz = 0, clk_khz = 0;
@@ -289,6 +321,14 @@ int lfpeaks_measure(int device, double *out, int nout)
          });
          out[4] = ms > 0 ? (double)grid * 256 * iters * 8 * 2.0 / (ms * 1e-3) / 1e12 : 0.0;
          out[5] = x.clock_ghz(clk, grid);
+      }
+      {
+         // MFMA / VALU overlap: 8 MFMAs (512 pipe cycles) and 128 FMAs
+         // (512 cycles) per round, alone and mixed, 2 waves per SIMD
+         const int grid = ncu * 2, iters = 4000;
+         out[11 + 4 * 16] = x.time2([&] { hipLaunchKernelGGL((lfpeaks::k_peak_mix<8, 0>), dim3(grid), dim3(256), 0, x.s, iters, 0.999999, sink); });
+         out[12 + 4 * 16] = x.time2([&] { hipLaunchKernelGGL((lfpeaks::k_peak_mix<0, 128>), dim3(grid), dim3(256), 0, x.s, iters, 0.999999, sink); });
+         out[13 + 4 * 16] = x.time2([&] { hipLaunchKernelGGL((lfpeaks::k_peak_mix<8, 128>), dim3(grid), dim3(256), 0, x.s, iters, 0.999999, sink); });
       }
       hipLaunchKernelGGL(lfpeaks::k_fill_zero, dim3(4096), dim3(256), 0, x.s, a, n2);
       const lfpeaks::pk_dv2 *va = reinterpret_cast<const lfpeaks::pk_dv2 *>(a);
