@@ -486,7 +486,7 @@ template <int NBUF, int WPEU, bool PRIO = false>
 hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
 {
    const int nqt = P.rc.nq + P.rm.nq;
-   const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + NBUF * SYM_STG) * sizeof(double);
+   const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + SYM_FCS + NBUF * SYM_STG) * sizeof(double);
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_sym<NBUF, WPEU, PRIO>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }

@@ -15,11 +15,16 @@
 //     Ae(ti, tj) = S(ti, tj) + P(ti, tj) - P(tj, ti)^T
 //     Ae(tj, ti) = S(ti, tj)^T + P(tj, ti) - P(ti, tj)^T
 //     Ae(td, td) = S(td, td) + P(td, td) - P(td, td)^T
-// 216 MFMAs per wavefront instead of 270 at p = 3 with k_par (cc rule 11
-// points, mass rule 24).  The K stream stages per mass point only the value
-// rows v (the curl rows enter through X c), and per row the symmetric
-// coefficient row CS and the cross row CP.  Masks holding only one of
-// CROSS0 / CROSS1 are not antisymmetric and go to the general kernels.
+// The K stream stages per mass point only the value rows v (the curl rows
+// enter through X c), and per row the symmetric coefficient row CS.  X has
+// rank 2: its first two rows are i wb (c0y, -c0x) times e_2^T (k_par x
+// couples only the z component of the curl into the transverse values), so
+//     v^T X c = (c0y v0 - c0x v1) (i wb c2) + v2 (X c)_2,
+// and P takes two staged rows per mass point (AP = (u, v2), CP = the two
+// coefficient rows) instead of three.  180 MFMAs per wavefront at p = 3
+// with k_par (cc rule 11 points, mass rule 24) instead of 270 for the full
+// matrix.  Masks holding only one of CROSS0 / CROSS1 are not antisymmetric
+// and go to the general kernels.
 #pragma once
 
 #include "lfasm_device.hpp"
@@ -28,7 +33,10 @@ namespace lfasm {
 
 constexpr int SYM_NP = 48;                        // 3 tiles of 16 columns
 constexpr int SYM_ROWS = 12;                      // 4 points x 3 rows per chunk
-constexpr int SYM_STG = 5 * SYM_ROWS * SYM_NP;    // A, CSr, CSi, CPr, CPi
+constexpr int SYM_PROWS = 8;                      // 4 points x 2 rows of the P operands
+// A, CSr, CSi (SYM_ROWS rows each), AP, CPr, CPi (SYM_PROWS rows each)
+constexpr int SYM_STG = (3 * SYM_ROWS + 3 * SYM_PROWS) * SYM_NP;
+constexpr int SYM_FCS = 2 * SYM_NP;               // face-transform pair per local index
 constexpr int SYM_THREADS = 192;                  // 3 wavefronts per element
 
 struct SymShape {
@@ -51,7 +59,7 @@ __device__ __forceinline__ SymShape sym_shape(const LocalParams &P)
 // t % 48 (192 threads, 4 points x 48 columns).
 struct SymItem {
    double h[6];
-   int q, k, row0;
+   int q, k, row0, prow0;
    bool valid, live, mass;
 };
 
@@ -70,6 +78,7 @@ __device__ __forceinline__ SymItem sym_load_item(const LocalParams &P, const Sym
    it.valid = q < nq;
    it.q = it.valid ? q : 0;
    it.row0 = qi * 3;
+   it.prow0 = qi * 2;
    it.live = it.valid && it.k < P.nd;
    if (it.live)
    {
@@ -90,14 +99,16 @@ __device__ __forceinline__ SymItem sym_load_item(const LocalParams &P, const Sym
 }
 
 // Staging rows of one item: A (shape rows), CS (symmetric coefficients),
-// CP (cross coefficients, mass rows with the cross terms only).
+// AP / CP (the two rank-one pairs of the cross coupling, mass rows with the
+// cross terms only).
 __device__ __forceinline__ void sym_form_item(const LocalParams &P, const double *ec,
                                               const SymShape &ss, const double *qcoef,
                                               const SymItem &it, double *stg)
 {
-   constexpr int RN = SYM_ROWS * SYM_NP;
-   double *A = stg, *CSr = stg + RN, *CSi = stg + 2 * RN, *CPr = stg + 3 * RN, *CPi = stg + 4 * RN;
-   const int k = it.k, r0 = it.row0;
+   constexpr int RN = SYM_ROWS * SYM_NP, RP = SYM_PROWS * SYM_NP;
+   double *A = stg, *CSr = stg + RN, *CSi = stg + 2 * RN;
+   double *AP = stg + 3 * RN, *CPr = AP + RP, *CPi = AP + 2 * RP;
+   const int k = it.k, r0 = it.row0, p0 = it.prow0;
    if (!it.mass)
    {
       const double *qc = qcoef + (size_t)it.q * 4;
@@ -140,16 +151,18 @@ __device__ __forceinline__ void sym_form_item(const LocalParams &P, const double
    CSi[(r0 + 2) * SYM_NP + k] = S2.y;
    if (ss.cross)
    {
-      // the c-terms of C0..C2 of mass_coef: X c
-      const double2 X0 = cmul(wb, make_double2(0.0, mc[8] * c2));
-      const double2 X1 = cmul(wb, make_double2(0.0, -mc[7] * c2));
+      // the c-terms of C0..C2 of mass_coef, X c = (i wb c0y c2, -i wb c0x c2,
+      // i wa (c0x c1 - c0y c0)), as two rank-one pairs:
+      // v0 (X c)_0 + v1 (X c)_1 = u (i wb c2) with u = c0y v0 - c0x v1
+      const double u = mc[8] * v0 - mc[7] * v1;
+      const double2 Y = make_double2(-wb.y * c2, wb.x * c2);
       const double2 X2 = cmul(wa, make_double2(0.0, -mc[8] * c0 + mc[7] * c1));
-      CPr[(r0 + 0) * SYM_NP + k] = X0.x;
-      CPi[(r0 + 0) * SYM_NP + k] = X0.y;
-      CPr[(r0 + 1) * SYM_NP + k] = X1.x;
-      CPi[(r0 + 1) * SYM_NP + k] = X1.y;
-      CPr[(r0 + 2) * SYM_NP + k] = X2.x;
-      CPi[(r0 + 2) * SYM_NP + k] = X2.y;
+      AP[(p0 + 0) * SYM_NP + k] = u;
+      AP[(p0 + 1) * SYM_NP + k] = v2;
+      CPr[(p0 + 0) * SYM_NP + k] = Y.x;
+      CPi[(p0 + 0) * SYM_NP + k] = Y.y;
+      CPr[(p0 + 1) * SYM_NP + k] = X2.x;
+      CPi[(p0 + 1) * SYM_NP + k] = X2.y;
    }
 }
 
@@ -168,26 +181,37 @@ __device__ __forceinline__ dbl4 sym_transpose(const dbl4 x, double *tr, int g4, 
    return y;
 }
 
-// Dual face transform (columns, then rows; solver/asm_gpu.inc:221-251) and
-// store of output tile (I, J) of element e.
-__device__ __forceinline__ void sym_tile_out(const LocalParams &P, dbl4 (&t)[2], int e, int io,
-                                             int I, int J, int g4, int m16)
+// Face-transform coefficients of element e for every local index d < 48:
+// fcs[2d] (own weight), fcs[2d + 1] (partner weight); (1, 0) off the faces.
+// Threads tid < SYM_NP of the workgroup.
+__device__ __forceinline__ void sym_face_coef(const LocalParams &P, int e, int tid, double *fcs)
+{
+   if (tid >= SYM_NP) { return; }
+   const int p = P.p, foff = 6 * p, ntd = p * (p - 1);
+   double cs = 1.0, cp = 0.0;
+   if (P.apply_te && p >= 2 && tid >= foff && tid < foff + 4 * ntd && tid < P.nd)
+   {
+      const double *Ti = P.TInv + 4 * P.fo[(size_t)e * 4 + (tid - foff) / ntd];
+      const bool even = ((tid - foff) & 1) == 0;
+      cs = even ? Ti[0] : Ti[3];
+      cp = even ? Ti[1] : Ti[2];
+   }
+   fcs[2 * tid] = cs;
+   fcs[2 * tid + 1] = cp;
+}
+
+// Dual face transform (columns, then rows; solver/asm_gpu.inc:221-251) with
+// the element's coefficients fcs, and store of output tile (I, J) of
+// element e.
+__device__ __forceinline__ void sym_tile_out(const LocalParams &P, const double *fcs, dbl4 (&t)[2],
+                                             int e, int io, int I, int J, int g4, int m16)
 {
    const int nd = P.nd, p = P.p;
    if (P.apply_te && p >= 2)
    {
-      const int foff = 6 * p, ntd = p * (p - 1);
-      const int *fo = P.fo + (size_t)e * 4;
       {
          const int col = 16 * J + m16;
-         double cs = 1.0, cp = 0.0;
-         if (col >= foff && col < foff + 4 * ntd)
-         {
-            const double *Ti = P.TInv + 4 * fo[(col - foff) / ntd];
-            const bool even = ((col - foff) & 1) == 0;
-            cs = even ? Ti[0] : Ti[3];
-            cp = even ? Ti[1] : Ti[2];
-         }
+         const double cs = fcs[2 * col], cp = fcs[2 * col + 1];
 #pragma unroll
          for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -202,14 +226,7 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, dbl4 (&t)[2],
       for (int v = 0; v < 4; ++v)
       {
          const int row = 16 * I + g4 + 4 * v;
-         double cs = 1.0, cp = 0.0;
-         if (row >= foff && row < foff + 4 * ntd)
-         {
-            const double *Ti = P.TInv + 4 * fo[(row - foff) / ntd];
-            const bool even = ((row - foff) & 1) == 0;
-            cs = even ? Ti[0] : Ti[3];
-            cp = even ? Ti[1] : Ti[2];
-         }
+         const double cs = fcs[2 * row], cp = fcs[2 * row + 1];
 #pragma unroll
          for (int c = 0; c < 2; ++c)
          {
@@ -254,7 +271,7 @@ __global__ void __launch_bounds__(SYM_THREADS)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
 k_local_mfma_sym(LocalParams P, int nqt)
 {
-   constexpr int RN = SYM_ROWS * SYM_NP;
+   constexpr int RN = SYM_ROWS * SYM_NP, RP = SYM_PROWS * SYM_NP;
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
    const int tid = threadIdx.x;
@@ -264,8 +281,10 @@ k_local_mfma_sym(LocalParams P, int nqt)
    extern __shared__ double smem[];
    double *ec = smem;
    double *qcoef = ec + EC_SIZE;
-   double *buf = qcoef + nqt * 4;
+   double *fcs = qcoef + nqt * 4;
+   double *buf = fcs + SYM_FCS;
    elem_setup(P, e, nqt, tid, SYM_THREADS, qcoef, ec);
+   sym_face_coef(P, e, tid, fcs);
    const SymShape ss = sym_shape(P);
    __syncthreads();
 
@@ -290,7 +309,8 @@ k_local_mfma_sym(LocalParams P, int nqt)
    for (int ch = 0; ch < nch; ++ch)
    {
       const double *A = buf + (NBUF == 2 ? (ch & 1) * SYM_STG : 0);
-      const double *CSr = A + RN, *CSi = A + 2 * RN, *CPr = A + 3 * RN, *CPi = A + 4 * RN;
+      const double *CSr = A + RN, *CSi = A + 2 * RN;
+      const double *AP = A + 3 * RN, *CPr = AP + RP, *CPi = AP + 2 * RP;
       const bool more = ch + 1 < nch;
       const SymItem nxt = sym_load_item(P, ss, more ? ch + 1 : ch, tid);
       const bool cross = ss.cross && ch >= ss.nch_cc;   // wave-uniform
@@ -299,15 +319,21 @@ k_local_mfma_sym(LocalParams P, int nqt)
       for (int ks = 0; ks < SYM_ROWS / 4; ++ks)
       {
          const int o = (ks * 4 + g4) * SYM_NP + m16;
-         const double ai = A[o + 16 * ti], aj = A[o + 16 * tj], ad = A[o + 16 * td];
+         const double ai = A[o + 16 * ti], ad = A[o + 16 * td];
          const double sjr = CSr[o + 16 * tj], sji = CSi[o + 16 * tj];
          const double sdr = CSr[o + 16 * td], sdi = CSi[o + 16 * td];
          sij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sjr, sij[0], 0, 0, 0);
          sij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sji, sij[1], 0, 0, 0);
          sdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdr, sdd[0], 0, 0, 0);
          sdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdi, sdd[1], 0, 0, 0);
-         if (cross)
+      }
+      if (cross)
+      {
+#pragma unroll
+         for (int ks = 0; ks < SYM_PROWS / 4; ++ks)
          {
+            const int o = (ks * 4 + g4) * SYM_NP + m16;
+            const double ai = AP[o + 16 * ti], aj = AP[o + 16 * tj], ad = AP[o + 16 * td];
             const double pjr = CPr[o + 16 * tj], pjq = CPi[o + 16 * tj];
             const double pir = CPr[o + 16 * ti], piq = CPi[o + 16 * ti];
             const double pdr = CPr[o + 16 * td], pdq = CPi[o + 16 * td];
@@ -339,9 +365,9 @@ k_local_mfma_sym(LocalParams P, int nqt)
       t2[c] = sijT + pji[c] - pijT;
       t3[c] = sdd[c] + pdd[c] - pddT;
    }
-   sym_tile_out(P, t1, e, io, ti, tj, g4, m16);
-   sym_tile_out(P, t2, e, io, tj, ti, g4, m16);
-   sym_tile_out(P, t3, e, io, td, td, g4, m16);
+   sym_tile_out(P, fcs, t1, e, io, ti, tj, g4, m16);
+   sym_tile_out(P, fcs, t2, e, io, tj, ti, g4, m16);
+   sym_tile_out(P, fcs, t3, e, io, td, td, g4, m16);
 }
 
 }  // namespace lfasm
