@@ -450,7 +450,7 @@ def _quad_kernel_name(selector: int, nd: int) -> str:
     if not 32 < nd <= 48:
         return f"selector {selector}"
     return {1: "k_local_generic<4, 512> + k_te", 2: "k_local_mfma_sym<1, 3>",
-            3: "k_local_mfma_sym<1, 4>", 4: "k_local_mfma_pipe<3, 1, 4>",
+            3: "k_local_mfma_sym<2, 3, true>", 4: "k_local_mfma_pipe<3, 1, 4>",
             5: "k_local_mfma<3, 1, 1, 2>", 6: "k_local_mfma_sym<2, 3>",
             7: "k_local_mfma_sym<1, 3, true>"}.get(selector, f"selector {selector}")
 

@@ -522,7 +522,7 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
    case 2 * 16 + 4: return launch_mfma_pipe<2, 2, 2>(h, P);
    case 2 * 16 + 5: return launch_mfma_pipe<2, 1, 2>(h, P);
    case 3 * 16 + 0: return sym_applies(P) ? launch_mfma_sym<1, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<1, 4>(h, P) : launch_mfma_pipe<3, 2, 2>(h, P);
+   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<2, 3, true>(h, P) : launch_mfma_pipe<3, 2, 2>(h, P);
    case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
    case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<2, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
