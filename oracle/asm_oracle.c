@@ -443,10 +443,10 @@ int64_t orc_build_csr_symbolic(int ne, int nd, int height, const int *dof,
 void orc_free(void *p) { free(p); }
 
 /* solver/asm_gpu.inc:463-478 */
-void orc_eliminate(int n, const int *Ip, const int *Jcol,
-                   const unsigned char *is_ess, double *vre, double *vim)
+void orc_eliminate_rows(int r0, int r1, const int *Ip, const int *Jcol,
+                        const unsigned char *is_ess, double *vre, double *vim)
 {
-   for (int r = 0; r < n; ++r)
+   for (int r = r0; r < r1; ++r)
    {
       const int row_ess = is_ess[r];
       for (int k = Ip[r]; k < Ip[r + 1]; ++k)
@@ -459,6 +459,12 @@ void orc_eliminate(int n, const int *Ip, const int *Jcol,
          }
       }
    }
+}
+
+void orc_eliminate(int n, const int *Ip, const int *Jcol,
+                   const unsigned char *is_ess, double *vre, double *vim)
+{
+   orc_eliminate_rows(0, n, Ip, Jcol, is_ess, vre, vim);
 }
 
 /* solver/asm_gpu.inc:481-493 (row-wise complex SpMV) and :556-564 (norms) */
@@ -580,9 +586,9 @@ void orc_rhs_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incide
  * is linear in eps, so dA_e/d eps_e is the TE-transformed mass element
  * matrix with eps = 1; db/d eps_e is the element RHS at unit eps - eps_bg;
  * local vectors are sign * gathered global entries. */
-void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
-                         const orc_incident *inc, int rhs_col,
-                         const double *lam, const double *u, double *g)
+void orc_eps_sensitivity_list(const orc_asm_data *ad, const orc_problem *pb,
+                              const orc_incident *inc, int rhs_col, const double *lam,
+                              const double *u, int nel, const int *elems, double *g)
 {
    const int nd = ad->nd;
    C *Ae = (C *)malloc(sizeof(C) * nd * nd);
@@ -590,8 +596,9 @@ void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
    double *pc = (double *)malloc(sizeof(double) * 3 * nd);
    double *pv = (double *)malloc(sizeof(double) * 3 * nd);
    const C one = 1.0;
-   for (int e = 0; e < ad->ne; ++e)
+   for (int i = 0; i < nel; ++i)
    {
+      const int e = elems ? elems[i] : i;
       reassemble_element_eps(ad, pb, e, ORC_MASS, 1, &one, Ae, pc, pv);
       const int use_src = inc && rhs_col >= 0;
       if (use_src) { orc_rhs_element(ad, pb, inc, e, rhs_col, 1, (double *)be); }
@@ -609,13 +616,20 @@ void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
          if (use_src) { row -= be[a]; }
          acc += la * row;
       }
-      g[2 * (size_t)e] = creal(acc);
-      g[2 * (size_t)e + 1] = cimag(acc);
+      g[2 * (size_t)i] = creal(acc);
+      g[2 * (size_t)i + 1] = cimag(acc);
    }
    free(Ae);
    free(be);
    free(pc);
    free(pv);
+}
+
+void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
+                         const orc_incident *inc, int rhs_col,
+                         const double *lam, const double *u, double *g)
+{
+   orc_eps_sensitivity_list(ad, pb, inc, rhs_col, lam, u, ad->ne, NULL, g);
 }
 
 /* ---- componentwise error scale -------------------------------------------

@@ -104,6 +104,9 @@ void orc_free(void *p);
 /* solver/asm_gpu.inc:463-478: zero essential rows/cols, unit diagonal. */
 void orc_eliminate(int n, const int *I, const int *Jcol,
                    const unsigned char *is_ess, double *vre, double *vim);
+/* the same over rows [r0, r1) only (full-size checks of row samples) */
+void orc_eliminate_rows(int r0, int r1, const int *I, const int *Jcol,
+                        const unsigned char *is_ess, double *vre, double *vim);
 
 /* solver/asm_gpu.inc:481-493,533-565: rel = ||A X - B|| / ||B|| with X, B
  * as 2n [re; im] vectors. */
@@ -145,6 +148,10 @@ void orc_rhs_all(const orc_asm_data *ad, const orc_problem *pb, const orc_incide
 void orc_eps_sensitivity(const orc_asm_data *ad, const orc_problem *pb,
                          const orc_incident *inc, int rhs_col,
                          const double *lam, const double *u, double *g);
+/* the same for the elements elems[0..nel) (g[i] for elems[i]) */
+void orc_eps_sensitivity_list(const orc_asm_data *ad, const orc_problem *pb,
+                              const orc_incident *inc, int rhs_col, const double *lam,
+                              const double *u, int nel, const int *elems, double *g);
 
 /* Componentwise error scales (test metric, no reference counterpart): each
  * entry's expression evaluated on the moduli of its factors, with every sum

@@ -236,6 +236,48 @@ class Oracle:
             raise RuntimeError("CSR position missing")
         return vre, vim
 
+    def row_blocks(self, blocks, nthreads: int = 1, mask: int = ALL, apply_te: bool = True):
+        """CSR values and componentwise scales of the row blocks [(r0, r1), ...]
+        only, for checks of row samples of meshes too large for a full
+        reassembly: the element matrices and scales of the blocks'
+        contributor elements (host threads), then the row scatter of
+        scatter_threaded over each block.  Returns (vre, vim, s), nnz arrays
+        valid on the blocks' value ranges (zero elsewhere; the full-size
+        element arrays are allocated lazily and only the contributors' pages
+        are written)."""
+        from concurrent.futures import ThreadPoolExecutor
+        ne, nd = self.ad.ne, self.ad.nd
+        dof = self.ad.dof
+        sel = np.zeros(ne, dtype=bool)
+        for r0, r1 in blocks:
+            sel |= ((dof >= r0) & (dof < r1)).any(axis=1)
+        elems = np.nonzero(sel)[0]
+        Ae = np.zeros((ne, nd, nd), dtype=np.complex128)
+        S = np.zeros((ne, nd, nd))
+        parts = np.array_split(elems, max(1, nthreads))
+
+        def run(part):
+            for e in part:
+                e = int(e)
+                self.L.orc_reassemble_range(ct.byref(self.data), ct.byref(self.prob), e, e + 1,
+                                           mask, 1 if apply_te else 0,
+                                           Ae.ctypes.data + e * nd * nd * 16)
+                self.L.orc_scale_range(ct.byref(self.data), ct.byref(self.prob), e, e + 1, mask,
+                                      1 if apply_te else 0, S.ctypes.data + e * nd * nd * 8)
+
+        with ThreadPoolExecutor(max(1, nthreads)) as ex:
+            list(ex.map(run, parts))
+        nnz = int(self.I[-1])
+        vre, vim, s = np.zeros(nnz), np.zeros(nnz), np.zeros(nnz)
+        for r0, r1 in blocks:
+            rc = self.L.orc_scatter_rows(ct.byref(self.data), _ptr(Ae), int(r0), int(r1),
+                                         _ptr(vre), _ptr(vim))
+            rc |= self.L.orc_scale_scatter_rows(ct.byref(self.data), _ptr(S), int(r0), int(r1),
+                                               _ptr(s))
+            if rc:
+                raise RuntimeError("CSR position missing")
+        return vre, vim, s
+
     # ---- componentwise error scales (see orc_scale_range)
     def scale_elements(self, elems, mask: int = ALL, apply_te: bool = True) -> np.ndarray:
         """Scales of the element matrices of the given elements, (n, nd, nd)."""
@@ -305,13 +347,36 @@ class Oracle:
                               1 if unit else 0, _ptr(b))
         return b
 
-    def eps_sensitivity(self, lam: np.ndarray, u: np.ndarray, rhs_col: int = -1) -> np.ndarray:
+    def eps_sensitivity(self, lam: np.ndarray, u: np.ndarray, rhs_col: int = -1,
+                        elems=None, nthreads: int = 1) -> np.ndarray:
+        """Sensitivity of every element (elems None, serial) or of the listed
+        elements, in list order (chunks of the list on host threads)."""
         lam = np.ascontiguousarray(lam, dtype=np.complex128)
         u = np.ascontiguousarray(u, dtype=np.complex128)
-        g = np.zeros(self.ad.ne, dtype=np.complex128)
         inc = ct.byref(self.inc) if rhs_col >= 0 else None
-        self.L.orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob), inc, rhs_col,
-                                  _ptr(lam), _ptr(u), _ptr(g))
+        if elems is None:
+            g = np.zeros(self.ad.ne, dtype=np.complex128)
+            self.L.orc_eps_sensitivity(ct.byref(self.data), ct.byref(self.prob), inc, rhs_col,
+                                      _ptr(lam), _ptr(u), _ptr(g))
+            return g
+        from concurrent.futures import ThreadPoolExecutor
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        g = np.zeros(len(elems), dtype=np.complex128)
+        L = self.L
+        L.orc_eps_sensitivity_list.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                               ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
+                                               ct.c_void_p]
+        cuts = [(len(elems) * i) // max(1, nthreads) for i in range(max(1, nthreads) + 1)]
+
+        def run(i):
+            i0, i1 = cuts[i], cuts[i + 1]
+            if i1 > i0:
+                L.orc_eps_sensitivity_list(ct.byref(self.data), ct.byref(self.prob), inc, rhs_col,
+                                           _ptr(lam), _ptr(u), i1 - i0,
+                                           elems.ctypes.data + 4 * i0, g.ctypes.data + 16 * i0)
+
+        with ThreadPoolExecutor(max(1, nthreads)) as ex:
+            list(ex.map(run, range(max(1, nthreads))))
         return g
 
 
@@ -336,6 +401,21 @@ def eliminate(I, Jcol, is_ess, vre, vim):
     Jcol = np.ascontiguousarray(Jcol, dtype=np.int32)
     ess = np.ascontiguousarray(is_ess, dtype=np.uint8)
     lib().orc_eliminate(len(I) - 1, _ptr(I), _ptr(Jcol), _ptr(ess), _ptr(vre), _ptr(vim))
+    return vre, vim
+
+
+def eliminate_rows(I, Jcol, is_ess, vre, vim, blocks):
+    """eliminate() applied to the row blocks [(r0, r1), ...] only (copies)."""
+    vre = np.array(vre, dtype=np.float64)
+    vim = np.array(vim, dtype=np.float64)
+    I = np.ascontiguousarray(I, dtype=np.int32)
+    Jcol = np.ascontiguousarray(Jcol, dtype=np.int32)
+    ess = np.ascontiguousarray(is_ess, dtype=np.uint8)
+    L = lib()
+    L.orc_eliminate_rows.argtypes = [ct.c_int, ct.c_int] + [ct.c_void_p] * 5
+    for r0, r1 in blocks:
+        L.orc_eliminate_rows(int(r0), int(r1), _ptr(I), _ptr(Jcol), _ptr(ess), _ptr(vre),
+                             _ptr(vim))
     return vre, vim
 
 

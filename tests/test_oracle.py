@@ -487,6 +487,47 @@ def test_threaded_oracle_equals_serial():
     assert np.array_equal(a[0], s1[0]) and np.array_equal(a[1], s1[1])
 
 
+def test_row_blocks_equal_full_reassembly():
+    """The row-sample forms used on meshes too large for a full reassembly:
+    values, scales and elimination of a few row blocks (only their
+    contributor elements reassembled) equal the full oracle's bit for bit on
+    those blocks."""
+    c, ad, I, J = case("grating", 3, 6.0, "small")
+    orc = O.Oracle(ad, c.problem, I, J)
+    vre, vim = orc.assemble()
+    s = orc.scale_csr(3)
+    n = ad.height
+    blocks = [(0, 37), (n // 2, n // 2 + 51), (n - 29, n)]
+    bre, bim, bs = orc.row_blocks(blocks, 3)
+    is_ess = np.zeros(n, np.uint8)
+    is_ess[ad.ess] = 1
+    ere, eim = O.eliminate(I, J, is_ess, vre, vim)
+    fre, fim = O.eliminate_rows(I, J, is_ess, bre, bim, blocks)
+    for r0, r1 in blocks:
+        k = slice(I[r0], I[r1])
+        assert np.array_equal(bre[k], vre[k]) and np.array_equal(bim[k], vim[k])
+        assert np.array_equal(bs[k], s[k])
+        assert np.array_equal(fre[k], ere[k]) and np.array_equal(fim[k], eim[k])
+    assert np.any(is_ess[blocks[0][0]:blocks[0][1]]) or np.any(is_ess[blocks[2][0]:blocks[2][1]])
+
+
+def test_sensitivity_element_list_equals_full():
+    """The element-list form (host threads) of the sensitivity gives the full
+    form's values bit for bit, in list order."""
+    from lithofem_amd.incident import incident_set
+    c, ad, I, J = case("hole", 2, 6.0, "small")
+    orc = O.Oracle(ad, c.problem, I, J)
+    orc.set_incident(incident_set([c.problem], [[(6.0, 30.0, "p")]]))
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
+    u = rng.standard_normal(ad.height) + 1j * rng.standard_normal(ad.height)
+    elems = rng.permutation(ad.ne)[: ad.ne // 3]
+    for col in (-1, 0):
+        full = orc.eps_sensitivity(lam, u, col)
+        assert np.array_equal(orc.eps_sensitivity(lam, u, col, elems=elems, nthreads=4),
+                              full[elems])
+
+
 def test_rhs_constant_field_equals_mass_action():
     """For a constant incident field E0 and source strength deps, the RHS is
     b = -deps * M_src g(E0), with M_src the unit-eps mass matrix of the source
