@@ -655,3 +655,14 @@ def test_solve_json_problem():
     for r in range(2):
         want = o2.rhs(r)
         assert np.abs(o3.rhs(r) - want).max() <= 1e-14 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("name,p", [("grating", 2), ("grating", 3), ("hole", 2), ("hole", 3)])
+def test_parity_cases_cover_every_face_orientation(name, p):
+    """The cases of the GPU parity tests hold every one of the six face
+    orientations (bitmask 63), so each TInv block of the face transform is
+    exercised; the reference's own check only observes {0, 1, 3, 5}
+    (fo coverage 43, tests/test_asm_v25.py:134-140)."""
+    from cases import case
+    _, ad, _, _ = case(name, p, 6.0, "small")
+    assert sum(1 << int(v) for v in np.unique(ad.fo)) == 63
