@@ -198,6 +198,11 @@ def test_tensor_modes_global(p, renumber):
         g.set_tensor_path(2)
         g.compute_local()
         Af = g.download_local()
+        # a scatter after the on-demand download still sees the fused path's
+        # element coefficients
+        g.scatter_eliminate(True)
+        vre, vim = g.download_values()
+        assert np.array_equal(vre, vals[2][0]) and np.array_equal(vim, vals[2][1])
         g.set_tensor_path(1)
         g.compute_local()
         Am = g.download_local()
