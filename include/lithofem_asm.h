@@ -167,7 +167,7 @@ int lfasm_download_contributors(lfasm_handle *h, int *rc_ptr, int *rc_list);
 int lfasm_compute_local(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te);
 
 /* lfasm_compute_local with an explicit kernel choice (0 = automatic:
- * FP64 MFMA tiles for nd >= 45, VALU otherwise; 1 = VALU kernel;
+ * FP64 MFMA tiles for nd > 16, VALU otherwise; 1 = VALU kernel;
  * 2 + v = MFMA launch variant v, v < LFASM_N_MFMA_VARIANTS; for
  * 32 < nd <= 48 variant 0 is the symmetric split Ae = S + P - P^T when
  * CROSS0 and CROSS1 are both on or both off).  No reference counterpart:
