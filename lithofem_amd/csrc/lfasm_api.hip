@@ -968,19 +968,19 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.rslot = h->rslot;
    const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
    const bool bytes = h->rslot != nullptr;
-   const void *kern = bytes ? (const void *)k_scatter_rows<true> : (const void *)k_scatter_rows<false>;
+   // nd <= 32: two contributors per load instruction
+   const bool pack2 = h->nd <= 32;
+   const void *kern = bytes ? (pack2 ? (const void *)k_scatter_rows<true, true>
+                                     : (const void *)k_scatter_rows<true, false>)
+                            : (pack2 ? (const void *)k_scatter_rows<false, true>
+                                     : (const void *)k_scatter_rows<false, false>);
    LF_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
    if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
-   if (bytes)
-   {
-      hipLaunchKernelGGL(k_scatter_rows<true>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
-                         h->stream, S);
-   }
-   else
-   {
-      hipLaunchKernelGGL(k_scatter_rows<false>, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
-                         h->stream, S);
-   }
+   const dim3 grid((h->height + 3) / 4);
+   if (bytes && pack2) { hipLaunchKernelGGL((k_scatter_rows<true, true>), grid, dim3(BLOCK), shmem, h->stream, S); }
+   else if (bytes) { hipLaunchKernelGGL((k_scatter_rows<true, false>), grid, dim3(BLOCK), shmem, h->stream, S); }
+   else if (pack2) { hipLaunchKernelGGL((k_scatter_rows<false, true>), grid, dim3(BLOCK), shmem, h->stream, S); }
+   else { hipLaunchKernelGGL((k_scatter_rows<false, false>), grid, dim3(BLOCK), shmem, h->stream, S); }
    LF_TRY(hipGetLastError());
    if (timed) { LF_TRY(ev_end(h, T_SCATTER)); }
    return hipSuccess;

@@ -976,6 +976,106 @@ __device__ __forceinline__ void scatter_group(const ScatterParams &S, double2 *b
    }
 }
 
+// nd <= 32: two contributors per load instruction -- lanes 0-31 carry
+// contributor cg + 2u, lanes 32-63 contributor cg + 2u + 1 -- for the
+// first m (<= 2 NP) contributors of the group starting at list entry cg;
+// lanes of a half without a contributor issue no loads.  The accumulation
+// still runs contributor by contributor in list order (deterministic sums,
+// the same as scatter_group's).  At p = 2 a row's 20 entries filled a third
+// of a load instruction's lanes.
+template <int NP, bool BYTE_SLOTS>
+__device__ __forceinline__ void scatter_group2(const ScatterParams &S, double2 *buf, int cg, int m,
+                                               int lane, unsigned ess_bit, const uint8_t *sblk)
+{
+   const int nd = S.nd;
+   const int half = lane >> 5;
+   const int b = lane & 31;
+   const int bc = min(b, nd - 1);      // lanes past the row load a valid entry, unused
+   int ea[NP];
+   bool valid[NP];
+   dbl2v x[NP];
+   int sl[NP];
+   unsigned fa[NP], fb[NP];
+#pragma unroll
+   for (int u = 0; u < NP; ++u)
+   {
+      valid[u] = 2 * u + half < m;
+      ea[u] = valid[u] ? S.rc_list[cg + 2 * u + half] : 0;
+   }
+#pragma unroll
+   for (int u = 0; u < NP; ++u)
+   {
+      x[u] = dbl2v{0.0, 0.0};
+      if (valid[u])
+      {
+         x[u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2v *>(S.Ae) +
+                                           (size_t)ea[u] * S.lda + bc);
+      }
+   }
+#pragma unroll
+   for (int u = 0; u < NP; ++u)
+   {
+      sl[u] = 0;
+      if (valid[u])
+      {
+         if constexpr (BYTE_SLOTS) { sl[u] = sblk[(2 * u + half) * nd + bc]; }
+         else { sl[u] = S.slot[(size_t)ea[u] * S.sld + bc]; }
+      }
+   }
+#pragma unroll
+   for (int u = 0; u < NP; ++u)
+   {
+      fa[u] = 0;
+      fb[u] = 2;   // treated as essential: never added
+      if (valid[u])
+      {
+         const int e = ea[u] / nd;
+         if (S.emask)
+         {
+            const uint64_t sm = S.emask[2 * (size_t)e], em = S.emask[2 * (size_t)e + 1];
+            fa[u] = (unsigned)((sm >> (ea[u] - e * nd)) & 1);
+            fb[u] = (unsigned)(((sm >> bc) & 1) | (((em >> bc) & 1) << 1));
+         }
+         else
+         {
+            fa[u] = S.eflag[ea[u]];
+            fb[u] = S.eflag[(size_t)e * nd + bc];
+         }
+      }
+   }
+#pragma unroll
+   for (int u = 0; u < NP; ++u)
+   {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+      {
+         if (half == h && valid[u] && b < nd && (fb[u] & ess_bit) == 0)
+         {
+            const bool neg = ((fa[u] ^ fb[u]) & 1) != 0;
+            double2 t = buf[sl[u]];
+            t.x += neg ? -x[u].x : x[u].x;
+            t.y += neg ? -x[u].y : x[u].y;
+            buf[sl[u]] = t;
+         }
+         wave_sync_lds();
+      }
+   }
+}
+
+// scatter_group2<np> for the pairs of a group of m <= 2 NP contributors
+// (np = ceil(m / 2), wave-uniform)
+template <int NP, bool BYTE_SLOTS>
+__device__ __forceinline__ void scatter_dispatch2(const ScatterParams &S, double2 *buf, int cg, int m,
+                                                  int lane, unsigned ess_bit, const uint8_t *sblk)
+{
+   if constexpr (NP == 1) { scatter_group2<1, BYTE_SLOTS>(S, buf, cg, m, lane, ess_bit, sblk); }
+   else
+   {
+      if ((m + 1) / 2 == NP) { scatter_group2<NP, BYTE_SLOTS>(S, buf, cg, m, lane, ess_bit, sblk); }
+      else { scatter_dispatch2<NP - 1, BYTE_SLOTS>(S, buf, cg, m, lane, ess_bit, sblk); }
+   }
+}
+
 // scatter_group<m> for a group of m <= M contributors (m is wave-uniform)
 template <int M, bool BYTE_SLOTS>
 __device__ __forceinline__ void scatter_dispatch(const ScatterParams &S, double2 *buf, int cg, int m,
@@ -989,7 +1089,10 @@ __device__ __forceinline__ void scatter_dispatch(const ScatterParams &S, double2
    }
 }
 
-template <bool BYTE_SLOTS>
+constexpr int SCATTER_NP = 4;   // contributor pairs per group of the packed form (nd <= 32)
+
+// PACK2 (nd <= 32): two contributors per load instruction (scatter_group2)
+template <bool BYTE_SLOTS, bool PACK2 = false>
 __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
 {
    extern __shared__ double2 rowbuf[];
@@ -1008,10 +1111,24 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
    {
       const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
       const uint8_t *sblk = BYTE_SLOTS ? S.rslot + S.rsoff[r] : nullptr;
-      for (int cg = c0; cg < c1; cg += SCATTER_CG)
+      if constexpr (PACK2)
       {
-         scatter_dispatch<SCATTER_CG, BYTE_SLOTS>(S, buf, cg, min(SCATTER_CG, c1 - cg), lane, ess_bit,
-                                                  BYTE_SLOTS ? sblk + (size_t)(cg - c0) * S.nd : nullptr);
+         for (int cg = c0; cg < c1; cg += 2 * SCATTER_NP)
+         {
+            scatter_dispatch2<SCATTER_NP, BYTE_SLOTS>(
+               S, buf, cg, min(2 * SCATTER_NP, c1 - cg), lane, ess_bit,
+               BYTE_SLOTS ? sblk + (size_t)(cg - c0) * S.nd : nullptr);
+         }
+      }
+      else
+      {
+         for (int cg = c0; cg < c1; cg += SCATTER_CG)
+         {
+            scatter_dispatch<SCATTER_CG, BYTE_SLOTS>(S, buf, cg, min(SCATTER_CG, c1 - cg), lane,
+                                                     ess_bit,
+                                                     BYTE_SLOTS ? sblk + (size_t)(cg - c0) * S.nd
+                                                                : nullptr);
+         }
       }
    }
    else
