@@ -177,7 +177,7 @@ int lfasm_compute_local_kernel(lfasm_handle *h, int mask_cc, int mask_mass, int 
                                int kernel);
 
 /* Number of MFMA launch variants (kernel selectors 2 .. 2+N-1). */
-#define LFASM_N_MFMA_VARIANTS 6
+#define LFASM_N_MFMA_VARIANTS 8
 
 /* Make `kernel` (a selector of lfasm_compute_local_kernel) the one that
  * selector 0 uses on this handle -- the launch autotuner's result.  0

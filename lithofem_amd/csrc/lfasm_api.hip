@@ -257,6 +257,7 @@ struct lfasm_handle {
    hipEvent_t ev[2 * T_N] = {};
    bool ev_used[T_N] = {};
    int local_kernel = 0;   // tuned choice used by kernel selector 0
+   int *sym_queue = nullptr;  // work counter of the persistent quadrature kernel
 
    int fail(int code, const char *fmt, ...)
    {
@@ -481,17 +482,39 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
 }
 
 // Symmetric-split kernel (nd <= 48 with 3 tiles; CROSS0 and CROSS1 both on
-// or both off).
-template <int NBUF, int WPEU, bool PRIO = false>
+// or both off), after its setup-record kernel.  PERSIST: as many workgroups
+// as fit on the device at once, fed from the handle's work counter.
+template <int NBUF, int WPEU, int PRIO = 0, bool PERSIST = false>
 hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
 {
    const int nqt = P.rc.nq + P.rm.nq;
+   // each entry's setup record sits in its own element-matrix slot
+   if ((size_t)sym_rec_size(nqt) > 2 * (size_t)P.nd * P.lda) { return hipErrorInvalidValue; }
+   // the persistent form prefetches a record in SYM_PF registers per thread
+   if (PERSIST && sym_rec_size(nqt) > SYM_PF * SYM_THREADS)
+   {
+      return launch_mfma_sym<NBUF, WPEU, PRIO, false>(h, P);
+   }
    const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + SYM_FCS + NBUF * SYM_STG) * sizeof(double);
-   hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_sym<NBUF, WPEU, PRIO>,
+   auto kern = k_local_mfma_sym<NBUF, WPEU, PRIO, PERSIST>;
+   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   hipLaunchKernelGGL((k_local_mfma_sym<NBUF, WPEU, PRIO>), dim3((unsigned)P.nout),
-                      dim3(SYM_THREADS), shmem, h->stream, P, nqt);
+   int grid = P.nout;
+   if (PERSIST)
+   {
+      int per_cu = 0;
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, SYM_THREADS,
+                                                       shmem);
+      if (e != hipSuccess) { return e; }
+      grid = std::max(1, std::min(P.nout, std::max(1, per_cu) * h->cus));
+   }
+   hipLaunchKernelGGL(k_sym_prep, dim3((unsigned)P.nout), dim3(64), 0, h->stream, P, nqt,
+                      PERSIST ? h->sym_queue : nullptr);
+   e = hipGetLastError();
+   if (e != hipSuccess) { return e; }
+   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(SYM_THREADS), shmem, h->stream, P, nqt,
+                      h->sym_queue);
    return hipGetLastError();
 }
 
@@ -515,24 +538,32 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
    case 1 * 16 + 3: return launch_mfma_pipe<1, 4, 4>(h, P);
    case 1 * 16 + 4: return launch_mfma_pipe<1, 4, 2>(h, P);
    case 1 * 16 + 5: return launch_mfma_pipe<1, 2, 2>(h, P);
+   case 1 * 16 + 6: return launch_mfma_pipe<1, 4, 2>(h, P);
+   case 1 * 16 + 7: return launch_mfma_pipe<1, 4, 2>(h, P);
    case 2 * 16 + 0: return launch_mfma<2, 2, 4, 2>(h, P);
    case 2 * 16 + 1: return launch_mfma_pipe<2, 1, 2>(h, P);
    case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P);
    case 2 * 16 + 3: return launch_mfma_pipe<2, 2, 2>(h, P);
    case 2 * 16 + 4: return launch_mfma_pipe<2, 2, 2>(h, P);
    case 2 * 16 + 5: return launch_mfma_pipe<2, 1, 2>(h, P);
+   case 2 * 16 + 6: return launch_mfma<2, 2, 4, 2>(h, P);
+   case 2 * 16 + 7: return launch_mfma<2, 2, 4, 2>(h, P);
    case 3 * 16 + 0: return sym_applies(P) ? launch_mfma_sym<1, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<2, 3, true>(h, P) : launch_mfma_pipe<3, 2, 2>(h, P);
+   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<2, 3, 1>(h, P) : launch_mfma_pipe<3, 2, 2>(h, P);
    case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
-   case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<2, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 5: return sym_applies(P) ? launch_mfma_sym<1, 3, true>(h, P) : launch_mfma_pipe<3, 1, 2>(h, P);
+   case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<1, 3, 2>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
+   case 3 * 16 + 5: return sym_applies(P) ? launch_mfma_sym<1, 3, 1>(h, P) : launch_mfma_pipe<3, 1, 2>(h, P);
+   case 3 * 16 + 6: return sym_applies(P) ? launch_mfma_sym<1, 3, 1, true>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
+   case 3 * 16 + 7: return sym_applies(P) ? launch_mfma_sym<1, 3, 2, true>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
    case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
    case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
    case 6 * 16 + 3: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 4: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 5: return launch_mfma_pipe<6, 1, 1>(h, P);
+   case 6 * 16 + 6: return launch_mfma_pipe<6, 1, 2>(h, P);
+   case 6 * 16 + 7: return launch_mfma_pipe<6, 1, 2>(h, P);
    default: return hipErrorInvalidValue;
    }
 }
@@ -1023,6 +1054,7 @@ int lfasm_create(int device, lfasm_handle **out)
    }
 
    for (int i = 0; i < 2 * T_N; ++i) { LF_HIP(hipEventCreate(&h->ev[i])); }
+   LF_HIP(hipMalloc((void **)&h->sym_queue, 256));
    return LFASM_OK;
 }
 
@@ -1043,6 +1075,7 @@ void lfasm_destroy(lfasm_handle *h)
       if (h->pin[k]) { (void)hipHostFree(h->pin[k]); }
       if (h->pin_ev[k]) { (void)hipEventDestroy(h->pin_ev[k]); }
    }
+   dfree(h->sym_queue);
    if (h->stream) { (void)hipStreamDestroy(h->stream); }
    delete h;
 }

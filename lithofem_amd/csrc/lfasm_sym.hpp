@@ -13,8 +13,10 @@
 // 16-wide tiles (nd <= 48) a wavefront owns the tile pair {(ti, tj), (tj, ti)}
 // and one diagonal tile td:
 //     Ae(ti, tj) = S(ti, tj) + P(ti, tj) - P(tj, ti)^T
-//     Ae(tj, ti) = S(ti, tj)^T + P(tj, ti) - P(ti, tj)^T
+//     Ae(tj, ti) = (S(ti, tj) + P(tj, ti)^T - P(ti, tj))^T
 //     Ae(td, td) = S(td, td) + P(td, td) - P(td, td)^T
+// P(tj, ti)^T is accumulated directly in the (ti, tj) orientation (the MFMA
+// operands swapped), so the pair needs one transpose.
 // The K stream stages per mass point only the value rows v (the curl rows
 // enter through X c), and per row the symmetric coefficient row CS.  X has
 // rank 2: its first two rows are i wb (c0y, -c0x) times e_2^T (k_par x
@@ -200,6 +202,38 @@ __device__ __forceinline__ void sym_face_coef(const LocalParams &P, int e, int t
    fcs[2 * tid + 1] = cp;
 }
 
+// Per-element setup of the symmetric kernel, formed ahead of it by
+// k_sym_prep: [element constants (EC_SIZE) | per-point factors (4 nqt) |
+// face-transform coefficients (SYM_FCS)] doubles, the kernel's LDS layout.
+// The record of list entry io is written into the first doubles of that
+// entry's own element-matrix slot in Ae, which the quadrature kernel reads
+// (and stages in LDS) before it writes the entry's matrix; the slot holds
+// nd * lda complex entries, far more than the record.  This takes the
+// dependent geometry/material loads, the PML profile (pow) and the
+// divisions off the MFMA kernel's critical path.
+__host__ __device__ constexpr int sym_rec_size(int nqt) { return EC_SIZE + 4 * nqt + SYM_FCS; }
+
+__device__ __forceinline__ double *sym_rec(const LocalParams &P, int io, int e)
+{
+   return reinterpret_cast<double *>(P.Ae + (size_t)(P.out_by_elem ? e : io) * P.nd * P.lda);
+}
+
+// One wavefront per list entry (nqt points over the 64 lanes); also resets
+// the work counter of the persistent quadrature kernel (when given).
+__global__ void __launch_bounds__(64) k_sym_prep(LocalParams P, int nqt, int *queue)
+{
+   const int io = blockIdx.x;
+   if (queue && io == 0 && threadIdx.x == 0) { *queue = 0; }
+   const int e = P.elist ? P.elist[io] : io;
+   const int tid = threadIdx.x;
+   double *rec = sym_rec(P, io, e);
+   ElemGeo g;
+   load_geo(P, e, g);
+   for (int q = tid; q < nqt; q += 64) { q_coef(P, g, q, rec + EC_SIZE + 4 * q); }
+   if (tid == 0) { store_elem_const(g, mass_coef(P, g.pm, g.eps), rec); }
+   sym_face_coef(P, e, tid, rec + EC_SIZE + 4 * nqt);
+}
+
 // Dual face transform (columns, then rows; solver/asm_gpu.inc:221-251) with
 // the element's coefficients fcs, and store of output tile (I, J) of
 // element e.
@@ -258,33 +292,45 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, const double 
    }
 }
 
-// One element per workgroup of 3 wavefronts.  NBUF = 2: double-buffered
-// staging as in k_local_mfma_pipe (the table loads of chunk c+1 are issued
-// before the MFMAs of chunk c, its rows formed after them); NBUF = 1: one
-// staging buffer (half the LDS, more workgroups per CU), the next chunk
-// formed between two barriers.
-// PRIO: the wavefronts raise their issue priority while they run the MFMAs
-// of a chunk and drop it for the staging, so that on a SIMD shared by
-// several elements the matrix pipe is fed first.
-template <int NBUF, int WAVES_PER_EU, bool PRIO = false>
+// One element per workgroup of 3 wavefronts at a time.  NBUF = 2:
+// double-buffered staging as in k_local_mfma_pipe (the table loads of chunk
+// c+1 are issued before the MFMAs of chunk c, its rows formed after them);
+// NBUF = 1: one staging buffer (half the LDS, more workgroups per CU), the
+// next chunk formed between two barriers.
+// PRIO = 1: the wavefronts raise their issue priority while they run the
+// MFMAs of a chunk and drop it for the staging, so that on a SIMD shared by
+// several elements the matrix pipe is fed first; PRIO = 2: also raised for
+// the epilogue (its FP64 face-transform FMAs share the SIMD's FP64 issue
+// with the other wavefronts' MFMAs).
+// PERSIST: a grid of a few workgroups per CU; each takes list entries from
+// `queue` (reset to 0 by k_sym_prep) and loads the next entry's setup record
+// into registers while it computes the current one, so neither the record
+// load nor a workgroup launch sits between two elements.
+constexpr int SYM_PF = 2;   // prefetch registers per thread (record <= SYM_PF * SYM_THREADS)
+
+template <int NBUF, int WAVES_PER_EU, int PRIO = 0, bool PERSIST = false>
 __global__ void __launch_bounds__(SYM_THREADS)
 __attribute__((amdgpu_waves_per_eu(WAVES_PER_EU)))
-k_local_mfma_sym(LocalParams P, int nqt)
+k_local_mfma_sym(LocalParams P, int nqt, int *queue)
 {
    constexpr int RN = SYM_ROWS * SYM_NP, RP = SYM_PROWS * SYM_NP;
    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-   const int lane = threadIdx.x & 63;
-   const int tid = threadIdx.x;
-   const int io = blockIdx.x;   // grid = nout
-   const int e = P.elist ? P.elist[io] : io;
+   __shared__ int s_next;
 
    extern __shared__ double smem[];
    double *ec = smem;
    double *qcoef = ec + EC_SIZE;
    double *fcs = qcoef + nqt * 4;
    double *buf = fcs + SYM_FCS;
-   elem_setup(P, e, nqt, tid, SYM_THREADS, qcoef, ec);
-   sym_face_coef(P, e, tid, fcs);
+   const int rs = sym_rec_size(nqt);
+   int io = blockIdx.x;   // grid <= nout
+   int e = P.elist ? P.elist[io] : io;
+   // the element's setup record (k_sym_prep), read before any output of
+   // this element is written over it
+   {
+      const double *rec = sym_rec(P, io, e);
+      for (int i = threadIdx.x; i < rs; i += SYM_THREADS) { ec[i] = rec[i]; }
+   }
    const SymShape ss = sym_shape(P);
    __syncthreads();
 
@@ -292,82 +338,125 @@ k_local_mfma_sym(LocalParams P, int nqt)
    const int ti = wave == 2 ? 1 : 0;
    const int tj = wave == 0 ? 1 : 2;
    const int td = 2 - wave;
-   dbl4 sij[2], pij[2], pji[2], sdd[2], pdd[2];
-#pragma unroll
-   for (int c = 0; c < 2; ++c)
-   {
-      sij[c] = dbl4{0.0, 0.0, 0.0, 0.0};
-      pij[c] = dbl4{0.0, 0.0, 0.0, 0.0};
-      pji[c] = dbl4{0.0, 0.0, 0.0, 0.0};
-      sdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
-      pdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
-   }
-   const int g4 = lane >> 4, m16 = lane & 15;
    const int nch = ss.nch_cc + ss.nch_m;
-   if (nch > 0) { sym_form_item(P, ec, ss, qcoef, sym_load_item(P, ss, 0, tid), buf); }
-   __syncthreads();
-   for (int ch = 0; ch < nch; ++ch)
+   for (;;)
    {
-      const double *A = buf + (NBUF == 2 ? (ch & 1) * SYM_STG : 0);
-      const double *CSr = A + RN, *CSi = A + 2 * RN;
-      const double *AP = A + 3 * RN, *CPr = AP + RP, *CPi = AP + 2 * RP;
-      const bool more = ch + 1 < nch;
-      const SymItem nxt = sym_load_item(P, ss, more ? ch + 1 : ch, tid);
-      const bool cross = ss.cross && ch >= ss.nch_cc;   // wave-uniform
-      if (PRIO) { __builtin_amdgcn_s_setprio(2); }
+      // the thread index made opaque to the loop over elements: without it
+      // the compiler keeps every per-thread address of the element's work
+      // live across the loop (spills)
+      int tid_v = threadIdx.x;
+      if (PERSIST) { asm volatile("" : "+v"(tid_v)); }
+      const int tid = tid_v;
+      const int lane = tid & 63;
+      const int g4 = lane >> 4, m16 = lane & 15;
+      if (PERSIST && tid == 0) { s_next = (int)gridDim.x + atomicAdd(queue, 1); }
+      dbl4 sij[2], pij[2], pjiT[2], sdd[2], pdd[2];
 #pragma unroll
-      for (int ks = 0; ks < SYM_ROWS / 4; ++ks)
+      for (int c = 0; c < 2; ++c)
       {
-         const int o = (ks * 4 + g4) * SYM_NP + m16;
-         const double ai = A[o + 16 * ti], ad = A[o + 16 * td];
-         const double sjr = CSr[o + 16 * tj], sji = CSi[o + 16 * tj];
-         const double sdr = CSr[o + 16 * td], sdi = CSi[o + 16 * td];
-         sij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sjr, sij[0], 0, 0, 0);
-         sij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sji, sij[1], 0, 0, 0);
-         sdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdr, sdd[0], 0, 0, 0);
-         sdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdi, sdd[1], 0, 0, 0);
+         sij[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+         pij[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+         pjiT[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+         sdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+         pdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
       }
-      if (cross)
-      {
+      if (nch > 0) { sym_form_item(P, ec, ss, qcoef, sym_load_item(P, ss, 0, tid), buf); }
+      __syncthreads();
+      // next list entry (read after the barrier that follows its fetch)
+      const int io_n = PERSIST ? __builtin_amdgcn_readfirstlane(s_next) : P.nout;
+      const bool has_next = io_n < P.nout;
+      double pf[SYM_PF];
 #pragma unroll
-         for (int ks = 0; ks < SYM_PROWS / 4; ++ks)
+      for (int k = 0; k < SYM_PF; ++k) { pf[k] = 0.0; }
+      if (PERSIST && has_next)
+      {
+         const double *rn = sym_rec(P, io_n, P.elist ? P.elist[io_n] : io_n);
+#pragma unroll
+         for (int k = 0; k < SYM_PF; ++k)
          {
-            const int o = (ks * 4 + g4) * SYM_NP + m16;
-            const double ai = AP[o + 16 * ti], aj = AP[o + 16 * tj], ad = AP[o + 16 * td];
-            const double pjr = CPr[o + 16 * tj], pjq = CPi[o + 16 * tj];
-            const double pir = CPr[o + 16 * ti], piq = CPi[o + 16 * ti];
-            const double pdr = CPr[o + 16 * td], pdq = CPi[o + 16 * td];
-            pij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, pjr, pij[0], 0, 0, 0);
-            pij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, pjq, pij[1], 0, 0, 0);
-            pji[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj, pir, pji[0], 0, 0, 0);
-            pji[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj, piq, pji[1], 0, 0, 0);
-            pdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdr, pdd[0], 0, 0, 0);
-            pdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdq, pdd[1], 0, 0, 0);
+            if (tid + k * SYM_THREADS < rs) { pf[k] = rn[tid + k * SYM_THREADS]; }
          }
       }
-      if (PRIO) { __builtin_amdgcn_s_setprio(0); }
-      if (NBUF == 1) { __syncthreads(); }
-      if (more) { sym_form_item(P, ec, ss, qcoef, nxt, buf + (NBUF == 2 ? ((ch + 1) & 1) * SYM_STG : 0)); }
-      __syncthreads();
-   }
-
-   // the staging is free now: per-wavefront transpose scratch
-   double *tr = buf + wave * 16 * 17;
-   dbl4 t1[2], t2[2], t3[2];
+      for (int ch = 0; ch < nch; ++ch)
+      {
+         const double *A = buf + (NBUF == 2 ? (ch & 1) * SYM_STG : 0);
+         const double *CSr = A + RN, *CSi = A + 2 * RN;
+         const double *AP = A + 3 * RN, *CPr = AP + RP, *CPi = AP + 2 * RP;
+         const bool more = ch + 1 < nch;
+         const SymItem nxt = sym_load_item(P, ss, more ? ch + 1 : ch, tid);
+         const bool cross = ss.cross && ch >= ss.nch_cc;   // wave-uniform
+         if (PRIO) { __builtin_amdgcn_s_setprio(2); }
 #pragma unroll
-   for (int c = 0; c < 2; ++c)
-   {
-      const dbl4 sijT = sym_transpose(sij[c], tr, g4, m16);
-      const dbl4 pijT = sym_transpose(pij[c], tr, g4, m16);
-      const dbl4 pjiT = sym_transpose(pji[c], tr, g4, m16);
-      const dbl4 pddT = sym_transpose(pdd[c], tr, g4, m16);
-      t1[c] = sij[c] + pij[c] - pjiT;
-      t2[c] = sijT + pji[c] - pijT;
-      t3[c] = sdd[c] + pdd[c] - pddT;
+         for (int ks = 0; ks < SYM_ROWS / 4; ++ks)
+         {
+            const int o = (ks * 4 + g4) * SYM_NP + m16;
+            const double ai = A[o + 16 * ti], ad = A[o + 16 * td];
+            const double sjr = CSr[o + 16 * tj], sji = CSi[o + 16 * tj];
+            const double sdr = CSr[o + 16 * td], sdi = CSi[o + 16 * td];
+            sij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sjr, sij[0], 0, 0, 0);
+            sij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, sji, sij[1], 0, 0, 0);
+            sdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdr, sdd[0], 0, 0, 0);
+            sdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, sdi, sdd[1], 0, 0, 0);
+         }
+         if (cross)
+         {
+#pragma unroll
+            for (int ks = 0; ks < SYM_PROWS / 4; ++ks)
+            {
+               const int o = (ks * 4 + g4) * SYM_NP + m16;
+               const double ai = AP[o + 16 * ti], aj = AP[o + 16 * tj], ad = AP[o + 16 * td];
+               const double pjr = CPr[o + 16 * tj], pjq = CPi[o + 16 * tj];
+               const double pir = CPr[o + 16 * ti], piq = CPi[o + 16 * ti];
+               const double pdr = CPr[o + 16 * td], pdq = CPi[o + 16 * td];
+               pij[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, pjr, pij[0], 0, 0, 0);
+               pij[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, pjq, pij[1], 0, 0, 0);
+               // P(tj, ti)^T in the (ti, tj) orientation: operands swapped
+               pjiT[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(pir, aj, pjiT[0], 0, 0, 0);
+               pjiT[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(piq, aj, pjiT[1], 0, 0, 0);
+               pdd[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdr, pdd[0], 0, 0, 0);
+               pdd[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, pdq, pdd[1], 0, 0, 0);
+            }
+         }
+         if (PRIO) { __builtin_amdgcn_s_setprio(0); }
+         if (NBUF == 1) { __syncthreads(); }
+         if (more)
+         {
+            sym_form_item(P, ec, ss, qcoef, nxt, buf + (NBUF == 2 ? ((ch + 1) & 1) * SYM_STG : 0));
+         }
+         __syncthreads();
+      }
+
+      // the staging is free now: per-wavefront transpose scratch
+      if (PRIO == 2) { __builtin_amdgcn_s_setprio(2); }
+      double *tr = buf + wave * 16 * 17;
+      // Ae(ti, tj) = S + P(ti, tj) - P(tj, ti)^T, and
+      // Ae(tj, ti) = (S + P(tj, ti)^T - P(ti, tj))^T: one transpose per pair
+      dbl4 t1[2], t2[2], t3[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+      {
+         t1[c] = sij[c] + pij[c] - pjiT[c];
+         t2[c] = sym_transpose(sij[c] + pjiT[c] - pij[c], tr, g4, m16);
+         const dbl4 pddT = sym_transpose(pdd[c], tr, g4, m16);
+         t3[c] = sdd[c] + pdd[c] - pddT;
+      }
+      sym_tile_out(P, fcs, t1, e, io, ti, tj, g4, m16);
+      sym_tile_out(P, fcs, t2, e, io, tj, ti, g4, m16);
+      sym_tile_out(P, fcs, t3, e, io, td, td, g4, m16);
+      if (PRIO == 2) { __builtin_amdgcn_s_setprio(0); }
+      if (!PERSIST || !has_next) { break; }
+      // every wavefront is done with this element's LDS (face coefficients,
+      // transpose scratch): the prefetched record replaces it
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < SYM_PF; ++k)
+      {
+         if (tid + k * SYM_THREADS < rs) { ec[tid + k * SYM_THREADS] = pf[k]; }
+      }
+      __syncthreads();
+      io = io_n;
+      e = P.elist ? P.elist[io] : io;
    }
-   sym_tile_out(P, fcs, t1, e, io, ti, tj, g4, m16);
-   sym_tile_out(P, fcs, t2, e, io, tj, ti, g4, m16);
-   sym_tile_out(P, fcs, t3, e, io, td, td, g4, m16);
 }
 
 }  // namespace lfasm

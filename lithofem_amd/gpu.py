@@ -31,7 +31,7 @@ RESIDUAL_STATIC_MAX_N = 1572864
 
 # kernel selector of lfasm_compute_local_kernel
 KERNEL_AUTO, KERNEL_VALU = 0, 1
-N_MFMA_VARIANTS = 6
+N_MFMA_VARIANTS = 8
 
 
 def kernel_mfma(variant: int = 0) -> int:
