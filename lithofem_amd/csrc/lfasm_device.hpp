@@ -460,6 +460,33 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const double *e
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2v __attribute__((ext_vector_type(2)));
 
+// Partner values of the face-pair transforms, moved by VALU lane
+// operations instead of the LDS crossbar (ds_bpermute): lane ^ 1 by a DPP
+// quad permutation [1, 0, 3, 2]; lane ^ 16 by v_permlane16_swap, which
+// swaps the odd 16-lane rows of its first operand with the even rows of its
+// second -- with the same register as both, the first result holds rows
+// [x0 x0 x2 x2] and the second [x1 x1 x3 x3], so a lane of an even row takes
+// the second and a lane of an odd row the first (lane layout checked on the
+// device).  Every lane of the wavefront must be active.
+__device__ __forceinline__ double lane_xor1(double x)
+{
+   const long long v = __double_as_longlong(x);
+   const int lo = __builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+   const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0xB1, 0xF, 0xF, false);
+   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double lane_xor16(double x)
+{
+   const long long v = __double_as_longlong(x);
+   const unsigned lo = (unsigned)v, hi = (unsigned)((unsigned long long)v >> 32);
+   const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+   const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+   const bool odd = ((__lane_id() >> 4) & 1) != 0;
+   const unsigned rl = odd ? pl[0] : pl[1], rh = odd ? ph[0] : ph[1];
+   return __longlong_as_double((long long)(((unsigned long long)rh << 32) | rl));
+}
+
 // Face transform and store of the MFMA accumulators of one wavefront
 // (row tiles wsub*RT .. wsub*RT+RT-1, all NT column tiles).
 template <int NT, int RT>
@@ -494,7 +521,7 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
                for (int v = 0; v < 4; ++v)
                {
                   const double x = acc[t][j][c][v];
-                  const double y = __shfl_xor(x, 1);
+                  const double y = lane_xor1(x);
                   acc[t][j][c][v] = cs * x + cp * y;
                }
       }
@@ -520,7 +547,7 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
                for (int c = 0; c < 2; ++c)
                {
                   const double x = acc[t][j][c][v];
-                  const double y = __shfl_xor(x, 16);
+                  const double y = lane_xor16(x);
                   acc[t][j][c][v] = cs * x + cp * y;
                }
          }

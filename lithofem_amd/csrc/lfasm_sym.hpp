@@ -252,7 +252,7 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, const double 
             for (int v = 0; v < 4; ++v)
             {
                const double x = t[c][v];
-               const double y = __shfl_xor(x, 1);
+               const double y = lane_xor1(x);
                t[c][v] = cs * x + cp * y;
             }
       }
@@ -265,7 +265,7 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, const double 
          for (int c = 0; c < 2; ++c)
          {
             const double x = t[c][v];
-            const double y = __shfl_xor(x, 16);
+            const double y = lane_xor16(x);
             t[c][v] = cs * x + cp * y;
          }
       }

@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P,
 #pragma unroll
       for (int s = 0; s < 2; ++s)
       {
-         const double ur = __shfl_xor(re[s], 1), ui = __shfl_xor(im[s], 1);
+         const double ur = lane_xor1(re[s]), ui = lane_xor1(im[s]);
          if (fc >= 0)
          {
             if (!odd)
