@@ -488,7 +488,10 @@ __device__ __forceinline__ double lane_xor16(double x)
 }
 
 // Face transform and store of the MFMA accumulators of one wavefront
-// (row tiles wsub*RT .. wsub*RT+RT-1, all NT column tiles).
+// (row tiles wsub*RT .. wsub*RT+RT-1, all NT column tiles), one 16 x 16
+// tile at a time: its columns, then its rows, then its store, so that only
+// one tile's partner values are live at once (a face DOF pair (d0, d0+1)
+// starts at an even index and never straddles two tiles).
 template <int NT, int RT>
 __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[RT][NT][2],
                                               int e, int io, bool live, int wsub, int g4,
@@ -497,24 +500,49 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
    // dual face-DOF transform TInv(fo)^T A TInv: columns, then rows
    // (solver/asm_gpu.inc:221-251)
    const int nd = P.nd, p = P.p;
-   if (P.apply_te && p >= 2)
+   const bool te = P.apply_te && p >= 2;
+   const int foff = 6 * p, ntd = p * (p - 1);
+   const int *fo = P.fo + (size_t)e * 4;
+   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * P.lda;
+   // rows padded to whole 128-byte lines: the 16 columns of a tile row are
+   // two whole lines (pad columns carry the zero accumulators), so the
+   // stores go out non-temporal; unpadded rows keep ordinary stores
+   const bool lines = (P.lda & 7) == 0;
+#pragma unroll
+   for (int t = 0; t < RT; ++t)
    {
-      const int foff = 6 * p, ntd = p * (p - 1);
-      const int *fo = P.fo + (size_t)e * 4;
+      // row weights of this row tile: row = 16 I + g4 + 4 v, so the pair
+      // (d0, d0+1) of a face sits in lanes l (g4 even) and l ^ 16 (g4
+      // odd), same register
+      double rcs[4], rcp[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+      {
+         const int row = 16 * (wsub * RT + t) + g4 + 4 * v;
+         rcs[v] = 1.0;
+         rcp[v] = 0.0;
+         if (te && row >= foff && row < foff + 4 * ntd)
+         {
+            const double *Ti = P.TInv + 4 * fo[(row - foff) / ntd];
+            const bool even = ((row - foff) & 1) == 0;
+            rcs[v] = even ? Ti[0] : Ti[3];
+            rcp[v] = even ? Ti[1] : Ti[2];
+         }
+      }
 #pragma unroll
       for (int j = 0; j < NT; ++j)
       {
          const int col = 16 * j + m16;
-         double cs = 1.0, cp = 0.0;
-         if (col >= foff && col < foff + 4 * ntd)
+         if (te)
          {
-            const double *Ti = P.TInv + 4 * fo[(col - foff) / ntd];
-            const bool even = ((col - foff) & 1) == 0;
-            cs = even ? Ti[0] : Ti[3];
-            cp = even ? Ti[1] : Ti[2];
-         }
-#pragma unroll
-         for (int t = 0; t < RT; ++t)
+            double cs = 1.0, cp = 0.0;
+            if (col >= foff && col < foff + 4 * ntd)
+            {
+               const double *Ti = P.TInv + 4 * fo[(col - foff) / ntd];
+               const bool even = ((col - foff) & 1) == 0;
+               cs = even ? Ti[0] : Ti[3];
+               cp = even ? Ti[1] : Ti[2];
+            }
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -524,47 +552,17 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
                   const double y = lane_xor1(x);
                   acc[t][j][c][v] = cs * x + cp * y;
                }
-      }
-      // rows: row = 16 I + g4 + 4 v, so the pair (d0, d0+1) of a face
-      // sits in lanes l (g4 even) and l ^ 16 (g4 odd), same register
 #pragma unroll
-      for (int t = 0; t < RT; ++t)
-#pragma unroll
-         for (int v = 0; v < 4; ++v)
-         {
-            const int row = 16 * (wsub * RT + t) + g4 + 4 * v;
-            double cs = 1.0, cp = 0.0;
-            if (row >= foff && row < foff + 4 * ntd)
-            {
-               const double *Ti = P.TInv + 4 * fo[(row - foff) / ntd];
-               const bool even = ((row - foff) & 1) == 0;
-               cs = even ? Ti[0] : Ti[3];
-               cp = even ? Ti[1] : Ti[2];
-            }
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
+            for (int v = 0; v < 4; ++v)
 #pragma unroll
                for (int c = 0; c < 2; ++c)
                {
                   const double x = acc[t][j][c][v];
                   const double y = lane_xor16(x);
-                  acc[t][j][c][v] = cs * x + cp * y;
+                  acc[t][j][c][v] = rcs[v] * x + rcp[v] * y;
                }
          }
-   }
-
-   if (!live) { return; }
-   double2 *Ae = P.Ae + (size_t)(P.out_by_elem ? e : io) * nd * P.lda;
-   // rows padded to whole 128-byte lines: the 16 columns of a tile row are
-   // two whole lines (pad columns carry the zero accumulators), so the
-   // stores go out non-temporal; unpadded rows keep ordinary stores
-   const bool lines = (P.lda & 7) == 0;
-#pragma unroll
-   for (int t = 0; t < RT; ++t)
-#pragma unroll
-      for (int j = 0; j < NT; ++j)
-      {
-         const int col = 16 * j + m16;
+         if (!live) { continue; }
 #pragma unroll
          for (int v = 0; v < 4; ++v)
          {
@@ -583,6 +581,7 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
             }
          }
       }
+   }
 }
 
 template <int NT, int RT, int EPB, int WAVES_PER_EU>

@@ -4,12 +4,20 @@ The reference extracts MFEM's ``IntRules.Get(TETRAHEDRON, 2p-2)`` and
 ``IntRules.Get(TETRAHEDRON, 2p)`` (solver/asm_data.inc:108-116) and hands the
 points and weights to the kernels as run-time tables.  MFEM is not available
 here, so this module produces rules of the same polynomial order with the
-same point counts MFEM's symmetric (Keast-type) rules have for orders 0-6
-(1/4/11/24 points).  The orbit parameters are refined by Newton iteration on
-the moment equations until every monomial of total degree <= order integrates
-to machine precision against the closed form i! j! k! / (i+j+k+3)!.
+same point counts MFEM's symmetric (Keast-type) rules have for orders 0-8
+(1/4/11/24/45 points).  The orbit parameters are refined by Newton iteration
+on the moment equations until every monomial of total degree <= order
+integrates to machine precision against the closed form
+i! j! k! / (i+j+k+3)!.
 
-Orders above 6 use a collapsed Gauss-Jacobi (Stroud conical product) rule.
+The degree-8 rule (the order-4 mass rule) is Keast's 45-point rule: the
+centroid with a negative weight, two S31, two S22 and two S211 orbits -- 15
+unknowns for the 15 S4-invariant moment conditions of degree <= 8.  Its
+starting values below came from a multi-start Newton search over the orbit
+parameters of that structure (the S31 and the first S22 parameters seeded at
+Keast's values); the solution's weights agree with Keast's table.
+
+Orders above 8 use a collapsed Gauss-Jacobi (Stroud conical product) rule.
 
 The kernels never assume a particular rule: points and weights are inputs,
 exactly as in the reference (solver/asm_gpu.inc:70-77).
@@ -127,6 +135,14 @@ _KEAST = {
          0.0406739585346113397, 0.00167953517588677620,
          0.322337890142275646, 0.00922619692394239843,
          0.0636610018750175299, 0.269672331458315867, 0.00803571428571428248]),
+    8: (["s4", "s31", "s31", "s22", "s22", "s211", "s211"],
+        [-0.039327006641137995,
+         0.12747093656672795, 0.0040813160593389216,
+         0.032078830392628654, 0.0006580867733034576,
+         0.049777095643258565, 0.004384258825121977,
+         0.31626955260150313, 0.013830063842480502,
+         0.037970048471853264, 0.7303134278075847, 0.00223873973961593,
+         0.2319010893971481, 0.5132800333608629, 0.004240437424685757]),
 }
 
 

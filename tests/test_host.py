@@ -138,6 +138,15 @@ def test_library_pml_smax(lib):
         assert abs(got - ref) <= 1e-15 * abs(ref)
 
 
+def test_tet_rule_point_counts():
+    """The symmetric rules have the point counts of MFEM's tetrahedron rules
+    for the orders the assembly asks for (2p - 2 and 2p, p = 1..4)."""
+    assert [len(tet_rule(o)[1]) for o in (0, 2, 4, 6, 8)] == [1, 4, 11, 24, 45]
+    # Keast's degree-8 rule carries one negative (centroid) weight
+    w8 = tet_rule(8)[1]
+    assert np.sum(w8 < 0) == 1 and abs(w8.sum() - 1.0 / 6.0) < 1e-16
+
+
 @pytest.mark.parametrize("order", range(0, 9))
 def test_tet_rule_is_exact_to_its_order(order):
     x, w = tet_rule(order)
