@@ -511,6 +511,10 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
 #pragma unroll
    for (int t = 0; t < RT; ++t)
    {
+      // a tile with no face DOF among its 16 indices has weights (1, 0)
+      // throughout: its pass is the identity and is skipped (wave-uniform)
+      const int I = wsub * RT + t;
+      const bool face_rows = te && 16 * I + 16 > foff && 16 * I < foff + 4 * ntd;
       // row weights of this row tile: row = 16 I + g4 + 4 v, so the pair
       // (d0, d0+1) of a face sits in lanes l (g4 even) and l ^ 16 (g4
       // odd), same register
@@ -533,7 +537,8 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
       for (int j = 0; j < NT; ++j)
       {
          const int col = 16 * j + m16;
-         if (te)
+         const bool face_cols = te && 16 * j + 16 > foff && 16 * j < foff + 4 * ntd;
+         if (face_cols)
          {
             double cs = 1.0, cp = 0.0;
             if (col >= foff && col < foff + 4 * ntd)
@@ -552,6 +557,9 @@ __device__ __forceinline__ void mfma_epilogue(const LocalParams &P, dbl4 (&acc)[
                   const double y = lane_xor1(x);
                   acc[t][j][c][v] = cs * x + cp * y;
                }
+         }
+         if (face_rows)
+         {
 #pragma unroll
             for (int v = 0; v < 4; ++v)
 #pragma unroll

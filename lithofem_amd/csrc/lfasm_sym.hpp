@@ -104,8 +104,8 @@ __device__ __forceinline__ SymItem sym_load_item(const LocalParams &P, const Sym
 // AP / CP (the two rank-one pairs of the cross coupling, mass rows with the
 // cross terms only).
 __device__ __forceinline__ void sym_form_item(const LocalParams &P, const double *ec,
-                                              const SymShape &ss, const double *qcoef,
-                                              const SymItem &it, double *stg)
+                                              const double *pj, const SymShape &ss,
+                                              const double *qcoef, const SymItem &it, double *stg)
 {
    constexpr int RN = SYM_ROWS * SYM_NP, RP = SYM_PROWS * SYM_NP;
    double *A = stg, *CSr = stg + RN, *CSi = stg + 2 * RN;
@@ -115,7 +115,7 @@ __device__ __forceinline__ void sym_form_item(const LocalParams &P, const double
    {
       const double *qc = qcoef + (size_t)it.q * 4;
       double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-      if (it.live) { piola_c(ec, it.h[0], it.h[1], it.h[2], c0, c1, c2); }
+      if (it.live) { piola_c(pj, it.h[0], it.h[1], it.h[2], c0, c1, c2); }
       const double war = it.valid ? qc[0] : 0.0, wai = it.valid ? qc[1] : 0.0;
       const double wbr = it.valid ? qc[2] : 0.0, wbi = it.valid ? qc[3] : 0.0;
       A[(r0 + 0) * SYM_NP + k] = c0;
@@ -133,8 +133,8 @@ __device__ __forceinline__ void sym_form_item(const LocalParams &P, const double
    double v0 = 0.0, v1 = 0.0, v2 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
    if (it.live)
    {
-      piola_v(ec, it.h[0], it.h[1], it.h[2], v0, v1, v2);
-      if (ss.cross) { piola_c(ec, it.h[3], it.h[4], it.h[5], c0, c1, c2); }
+      piola_v(pj, it.h[0], it.h[1], it.h[2], v0, v1, v2);
+      if (ss.cross) { piola_c(pj, it.h[3], it.h[4], it.h[5], c0, c1, c2); }
    }
    const double2 wa = it.valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
    const double2 wb = it.valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
@@ -241,8 +241,12 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, const double 
                                              int e, int io, int I, int J, int g4, int m16)
 {
    const int nd = P.nd, p = P.p;
-   if (P.apply_te && p >= 2)
+   const int foff = 6 * p, fend = 6 * p + 4 * p * (p - 1);
+   const bool te = P.apply_te && p >= 2;
+   const bool face_cols = te && 16 * J + 16 > foff && 16 * J < fend;
+   const bool face_rows = te && 16 * I + 16 > foff && 16 * I < fend;
    {
+      if (face_cols)
       {
          const int col = 16 * J + m16;
          const double cs = fcs[2 * col], cp = fcs[2 * col + 1];
@@ -256,17 +260,20 @@ __device__ __forceinline__ void sym_tile_out(const LocalParams &P, const double 
                t[c][v] = cs * x + cp * y;
             }
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
+      if (face_rows)
       {
-         const int row = 16 * I + g4 + 4 * v;
-         const double cs = fcs[2 * row], cp = fcs[2 * row + 1];
 #pragma unroll
-         for (int c = 0; c < 2; ++c)
+         for (int v = 0; v < 4; ++v)
          {
-            const double x = t[c][v];
-            const double y = lane_xor16(x);
-            t[c][v] = cs * x + cp * y;
+            const int row = 16 * I + g4 + 4 * v;
+            const double cs = fcs[2 * row], cp = fcs[2 * row + 1];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+            {
+               const double x = t[c][v];
+               const double y = lane_xor16(x);
+               t[c][v] = cs * x + cp * y;
+            }
          }
       }
    }
@@ -332,6 +339,13 @@ k_local_mfma_sym(LocalParams P, int nqt, int *queue)
       for (int i = threadIdx.x; i < rs; i += SYM_THREADS) { ec[i] = rec[i]; }
    }
    const SymShape ss = sym_shape(P);
+   // the element's Piola constants (J / detJ, J^-1) in scalar registers
+   double pj[18];
+   {
+      const double *rec = sym_rec(P, io, e);
+#pragma unroll
+      for (int i = 0; i < 18; ++i) { pj[i] = rec[i]; }
+   }
    __syncthreads();
 
    // tiles of this wavefront: the pair (ti, tj) and the diagonal td
@@ -360,7 +374,7 @@ k_local_mfma_sym(LocalParams P, int nqt, int *queue)
          sdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
          pdd[c] = dbl4{0.0, 0.0, 0.0, 0.0};
       }
-      if (nch > 0) { sym_form_item(P, ec, ss, qcoef, sym_load_item(P, ss, 0, tid), buf); }
+      if (nch > 0) { sym_form_item(P, ec, pj, ss, qcoef, sym_load_item(P, ss, 0, tid), buf); }
       __syncthreads();
       // next list entry (read after the barrier that follows its fetch)
       const int io_n = PERSIST ? __builtin_amdgcn_readfirstlane(s_next) : P.nout;
@@ -421,7 +435,7 @@ k_local_mfma_sym(LocalParams P, int nqt, int *queue)
          if (NBUF == 1) { __syncthreads(); }
          if (more)
          {
-            sym_form_item(P, ec, ss, qcoef, nxt, buf + (NBUF == 2 ? ((ch + 1) & 1) * SYM_STG : 0));
+            sym_form_item(P, ec, pj, ss, qcoef, nxt, buf + (NBUF == 2 ? ((ch + 1) & 1) * SYM_STG : 0));
          }
          __syncthreads();
       }
