@@ -429,6 +429,16 @@ LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_
    return P;
 }
 
+// Setup records of the general MFMA kernels (k_elem_prep), one wavefront
+// per list entry; each record sits in its entry's own element-matrix slot.
+hipError_t launch_elem_prep(lfasm_handle *h, const LocalParams &P)
+{
+   const int nqt = P.rc.nq + P.rm.nq;
+   if ((size_t)(EC_SIZE + 4 * nqt) > 2 * (size_t)P.nd * P.lda) { return hipErrorInvalidValue; }
+   hipLaunchKernelGGL(k_elem_prep, dim3((unsigned)P.nout), dim3(64), 0, h->stream, P, nqt);
+   return hipGetLastError();
+}
+
 template <int NT, int RT, int EPB, int WPEU>
 hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P)
 {
@@ -440,6 +450,7 @@ hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma<NT, RT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
+   LF_TRY(launch_elem_prep(h, P));
    const int grid = (P.nout + EPB - 1) / EPB;
    hipLaunchKernelGGL((k_local_mfma<NT, RT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
                       h->stream, P, nqt);
@@ -475,6 +486,7 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_pipe<NT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
+   LF_TRY(launch_elem_prep(h, P));
    const int grid = (P.nout + EPB - 1) / EPB;
    hipLaunchKernelGGL((k_local_mfma_pipe<NT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
                       h->stream, P, nqt);

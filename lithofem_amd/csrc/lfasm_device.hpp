@@ -239,13 +239,13 @@ __device__ __forceinline__ void piola_v(const double *ec, double a0, double a1, 
 // Form one (point, basis k) item of a chunk into the LDS staging rows
 // A[row][k], Cr[row][k], Ci[row][k] (row stride NP).  qc = per-point
 // factors (wa.re, wa.im, wb.re, wb.im).  valid == false writes zero rows.
-__device__ __forceinline__ void form_cc_raw(const double *ec, const double *qc, bool valid,
+__device__ __forceinline__ void form_cc_raw(const double *pj, const double *qc, bool valid,
                                             bool live, double h0, double h1, double h2,
                                             double *A, double *Cr, double *Ci, int row0, int k,
                                             int NP)
 {
    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-   if (live) { piola_c(ec, h0, h1, h2, c0, c1, c2); }
+   if (live) { piola_c(pj, h0, h1, h2, c0, c1, c2); }
    const double war = valid ? qc[0] : 0.0, wai = valid ? qc[1] : 0.0;
    const double wbr = valid ? qc[2] : 0.0, wbi = valid ? qc[3] : 0.0;
    A[(row0 + 0) * NP + k] = c0;
@@ -259,7 +259,7 @@ __device__ __forceinline__ void form_cc_raw(const double *ec, const double *qc, 
    Ci[(row0 + 2) * NP + k] = wbi * c2;
 }
 
-__device__ __forceinline__ void form_cc_item(const LocalParams &P, const double *ec,
+__device__ __forceinline__ void form_cc_item(const LocalParams &P, const double *pj,
                                              const double *qc, int q, int k, bool valid,
                                              double *A, double *Cr, double *Ci, int row0,
                                              int NP)
@@ -273,19 +273,19 @@ __device__ __forceinline__ void form_cc_item(const LocalParams &P, const double 
       h1 = cs[1];
       h2 = cs[2];
    }
-   form_cc_raw(ec, qc, valid, live, h0, h1, h2, A, Cr, Ci, row0, k, NP);
+   form_cc_raw(pj, qc, valid, live, h0, h1, h2, A, Cr, Ci, row0, k, NP);
 }
 
-__device__ __forceinline__ void form_mass_raw(const double *ec, const double *qc, bool valid,
-                                              bool live, int rows, const double *hv,
-                                              const double *hc, double *A, double *Cr,
-                                              double *Ci, int row0, int k, int NP)
+__device__ __forceinline__ void form_mass_raw(const double *pj, const double *ec,
+                                              const double *qc, bool valid, bool live, int rows,
+                                              const double *hv, const double *hc, double *A,
+                                              double *Cr, double *Ci, int row0, int k, int NP)
 {
    double v0 = 0.0, v1 = 0.0, v2 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
    if (live)
    {
-      piola_v(ec, hv[0], hv[1], hv[2], v0, v1, v2);
-      if (rows == 6) { piola_c(ec, hc[0], hc[1], hc[2], c0, c1, c2); }
+      piola_v(pj, hv[0], hv[1], hv[2], v0, v1, v2);
+      if (rows == 6) { piola_c(pj, hc[0], hc[1], hc[2], c0, c1, c2); }
    }
    const double2 wa = valid ? make_double2(qc[0], qc[1]) : make_double2(0.0, 0.0);
    const double2 wb = valid ? make_double2(qc[2], qc[3]) : make_double2(0.0, 0.0);
@@ -321,8 +321,9 @@ __device__ __forceinline__ void form_mass_raw(const double *ec, const double *qc
    }
 }
 
-__device__ __forceinline__ void form_mass_item(const LocalParams &P, const double *ec,
-                                               const double *qc, int q, int k, bool valid,
+__device__ __forceinline__ void form_mass_item(const LocalParams &P, const double *pj,
+                                               const double *ec, const double *qc, int q, int k,
+                                               bool valid,
                                                int rows, double *A, double *Cr, double *Ci,
                                                int row0, int NP)
 {
@@ -342,7 +343,7 @@ __device__ __forceinline__ void form_mass_item(const LocalParams &P, const doubl
          hc[2] = cs[2];
       }
    }
-   form_mass_raw(ec, qc, valid, live, rows, hv, hc, A, Cr, Ci, row0, k, NP);
+   form_mass_raw(pj, ec, qc, valid, live, rows, hv, hc, A, Cr, Ci, row0, k, NP);
 }
 
 // One item (point, basis k) of chunk `ch` per thread, split into the
@@ -392,20 +393,22 @@ __device__ __forceinline__ ChunkItem load_chunk_item(const LocalParams &P, const
    return it;
 }
 
-__device__ __forceinline__ void form_chunk_item(const LocalParams &P, const double *ec,
-                                                const StreamShape &ss, const double *qcoef,
-                                                const ChunkItem &it, double *A, double *Cr,
-                                                double *Ci, int NP)
+// pj: the element's Piola constants (J / detJ, J^-1; registers in the MFMA
+// kernels), ec: the element constants in LDS (mass coefficients).
+__device__ __forceinline__ void form_chunk_item(const LocalParams &P, const double *pj,
+                                                const double *ec, const StreamShape &ss,
+                                                const double *qcoef, const ChunkItem &it,
+                                                double *A, double *Cr, double *Ci, int NP)
 {
    if (!it.has) { return; }
    if (!it.mass)
    {
-      form_cc_raw(ec, qcoef + (size_t)it.q * 4, it.valid, it.live, it.h[0], it.h[1], it.h[2],
+      form_cc_raw(pj, qcoef + (size_t)it.q * 4, it.valid, it.live, it.h[0], it.h[1], it.h[2],
                   A, Cr, Ci, it.row0, it.k, NP);
    }
    else
    {
-      form_mass_raw(ec, qcoef + (size_t)(P.rc.nq + it.q) * 4, it.valid, it.live, ss.rows_m,
+      form_mass_raw(pj, ec, qcoef + (size_t)(P.rc.nq + it.q) * 4, it.valid, it.live, ss.rows_m,
                     it.h, it.h + 3, A, Cr, Ci, it.row0, it.k, NP);
    }
 }
@@ -413,8 +416,9 @@ __device__ __forceinline__ void form_chunk_item(const LocalParams &P, const doub
 // Form chunk `ch` of the combined K stream (cc chunks first, then mass
 // chunks) for the element group: items (point, basis) strided over nthr
 // threads starting at tid.  Staging: A, Cr, Ci of CHUNK_ROWS x NP each.
-__device__ __forceinline__ void form_chunk(const LocalParams &P, const double *ec,
-                                           const StreamShape &ss, const double *qcoef,
+__device__ __forceinline__ void form_chunk(const LocalParams &P, const double *pj,
+                                           const double *ec, const StreamShape &ss,
+                                           const double *qcoef,
                                            int ch, int tid, int nthr, double *A, double *Cr,
                                            double *Ci, int NP)
 {
@@ -427,7 +431,7 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const double *e
          const int q = ch * ss.qpc_cc + qi;
          const bool valid = q < ss.nq_cc;
          const int qs = valid ? q : 0;
-         form_cc_item(P, ec, qcoef + (size_t)qs * 4, qs, k, valid, A, Cr, Ci, qi * 3, NP);
+         form_cc_item(P, pj, qcoef + (size_t)qs * 4, qs, k, valid, A, Cr, Ci, qi * 3, NP);
       }
    }
    else
@@ -440,7 +444,7 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const double *e
          const int q = c * ss.qpc_m + qi;
          const bool valid = q < ss.nq_m;
          const int qs = valid ? q : 0;
-         form_mass_item(P, ec, qcoef + (size_t)(P.rc.nq + qs) * 4, qs, k, valid, ss.rows_m,
+         form_mass_item(P, pj, ec, qcoef + (size_t)(P.rc.nq + qs) * 4, qs, k, valid, ss.rows_m,
                         A, Cr, Ci, qi * ss.rows_m, NP);
       }
    }
@@ -459,6 +463,26 @@ __device__ __forceinline__ void form_chunk(const LocalParams &P, const double *e
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2v __attribute__((ext_vector_type(2)));
+
+// Per-element setup record of the MFMA quadrature kernels, formed ahead of
+// them by a one-wavefront-per-element kernel: [element constants (EC_SIZE) |
+// per-point factors (4 per rule point)] doubles, the kernels' LDS layout.
+// The record of list entry io sits in the first doubles of that entry's
+// own element-matrix slot in Ae (nd * lda complex entries), which the
+// quadrature kernel reads before it writes the entry's matrix: the
+// dependent geometry/material loads, the PML profile (pow) and the
+// divisions leave the MFMA kernels.
+__device__ __forceinline__ double *elem_rec(const LocalParams &P, int io, int e)
+{
+   return reinterpret_cast<double *>(P.Ae + (size_t)(P.out_by_elem ? e : io) * P.nd * P.lda);
+}
+
+__global__ void __launch_bounds__(64) k_elem_prep(LocalParams P, int nqt)
+{
+   const int io = blockIdx.x;
+   const int e = P.elist ? P.elist[io] : io;
+   elem_setup(P, e, nqt, threadIdx.x, 64, elem_rec(P, io, e) + EC_SIZE, elem_rec(P, io, e));
+}
 
 // Partner values of the face-pair transforms, moved by VALU lane
 // operations instead of the LDS crossbar (ds_bpermute): lane ^ 1 by a DPP
@@ -617,7 +641,15 @@ k_local_mfma(LocalParams P, int nqt)
    double *Ci = Cr + CHUNK_ROWS * NP;
    const int gtid = wsub * 64 + lane;   // thread index within the element group
 
-   elem_setup(P, e, nqt, gtid, WPE * 64, qcoef, ec);
+   // the element's setup record (k_elem_prep) into LDS, its Piola constants
+   // into scalar registers
+   double pj[18];
+   {
+      const double *rec = elem_rec(P, io, e);
+      for (int i = gtid; i < EC_SIZE + 4 * nqt; i += WPE * 64) { ec[i] = rec[i]; }
+#pragma unroll
+      for (int i = 0; i < 18; ++i) { pj[i] = rec[i]; }
+   }
    const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
    __syncthreads();
 
@@ -635,7 +667,7 @@ k_local_mfma(LocalParams P, int nqt)
    const int nch = ss.nch_cc + ss.nch_m;
    for (int ch = 0; ch < nch; ++ch)
    {
-      form_chunk(P, ec, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
+      form_chunk(P, pj, ec, ss, qcoef, ch, gtid, WPE * 64, A, Cr, Ci, NP);
       __syncthreads();
 #pragma unroll 1
       for (int ks = 0; ks < CHUNK_ROWS / 4; ++ks)
@@ -692,7 +724,15 @@ k_local_mfma_pipe(LocalParams P, int nqt)
    double *buf = qcoef + nqt * 4;
    const int gtid = wsub * 64 + lane;
 
-   elem_setup(P, e, nqt, gtid, NT * 64, qcoef, ec);
+   // the element's setup record (k_elem_prep) into LDS, its Piola constants
+   // into scalar registers
+   double pj[18];
+   {
+      const double *rec = elem_rec(P, io, e);
+      for (int i = gtid; i < EC_SIZE + 4 * nqt; i += NT * 64) { ec[i] = rec[i]; }
+#pragma unroll
+      for (int i = 0; i < 18; ++i) { pj[i] = rec[i]; }
+   }
    const StreamShape ss = stream_shape(P.use_cc, P.mflags, P.rc.nq, P.rm.nq);
    __syncthreads();
 
@@ -707,7 +747,7 @@ k_local_mfma_pipe(LocalParams P, int nqt)
    const int nch = ss.nch_cc + ss.nch_m;
    if (nch > 0)
    {
-      form_chunk(P, ec, ss, qcoef, 0, gtid, NT * 64, buf, buf + CHUNK_ROWS * NP,
+      form_chunk(P, pj, ec, ss, qcoef, 0, gtid, NT * 64, buf, buf + CHUNK_ROWS * NP,
                  buf + 2 * CHUNK_ROWS * NP, NP);
    }
    __syncthreads();
@@ -742,7 +782,7 @@ k_local_mfma_pipe(LocalParams P, int nqt)
       if (more)
       {
          double *An = buf + ((ch + 1) & 1) * STG;
-         form_chunk_item(P, ec, ss, qcoef, nxt, An, An + CHUNK_ROWS * NP,
+         form_chunk_item(P, pj, ec, ss, qcoef, nxt, An, An + CHUNK_ROWS * NP,
                          An + 2 * CHUNK_ROWS * NP, NP);
       }
       __syncthreads();
@@ -781,7 +821,7 @@ k_local_generic(LocalParams P, int nqt, int NP)
    const int nch = ss.nch_cc + ss.nch_m;
    for (int ch = 0; ch < nch; ++ch)
    {
-      form_chunk(P, ec, ss, qcoef, ch, tid, NTHR, A, Cr, Ci, NP);
+      form_chunk(P, ec, ec, ss, qcoef, ch, tid, NTHR, A, Cr, Ci, NP);
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < NACC; ++i)

@@ -215,7 +215,7 @@ __host__ __device__ constexpr int sym_rec_size(int nqt) { return EC_SIZE + 4 * n
 
 __device__ __forceinline__ double *sym_rec(const LocalParams &P, int io, int e)
 {
-   return reinterpret_cast<double *>(P.Ae + (size_t)(P.out_by_elem ? e : io) * P.nd * P.lda);
+   return elem_rec(P, io, e);
 }
 
 // One wavefront per list entry (nqt points over the 64 lanes); also resets
