@@ -9,7 +9,12 @@ elimination (lfasm_scatter_eliminate), inputs resident in HBM.
 
 Element matrices of constant-coefficient elements (outside the PML) come
 from the reference-tensor kernel (HBM-write bound), those of PML elements
-from the FP64 quadrature kernel (MFMA), then the scatter (HBM bound).
+from the FP64 quadrature kernel (MFMA), then the scatter (HBM bound).  The
+autotuner picks the quadrature kernel and whether the two element kernels
+run side by side (the tensor kernel held to a few workgroups per CU on one
+stream, the quadrature kernel on a second stream); the per-kernel figures
+come from eager steps with the kernels in sequence, `ms_per_step` from the
+captured step with the tuned setting.
 
 Prints one JSON line.  `value` is elements/s of the whole step; `roofline`
 is the kernel with the largest share of the step against its own roofline
@@ -171,6 +176,9 @@ def main() -> int:
     ap.add_argument("--tensor", default="on", choices=["on", "off", "fused"],
                     help="constant-coefficient elements: reference-tensor kernel (on), "
                          "quadrature kernels (off), tensor rows inside the scatter (fused)")
+    ap.add_argument("--concurrency", type=int, default=None,
+                    help="workgroup cap of the tensor kernel beside the quadrature kernel in "
+                         "the captured step (0 = in sequence; default: the autotuner's choice)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-variant and adjoint figures of the default run")
     ap.add_argument("--mode", default="assemble",
@@ -244,14 +252,25 @@ def main() -> int:
     log(f"element classes: {nconst} constant-coefficient (reference-tensor kernel), "
         f"{nquad} PML (quadrature kernel)")
     tuned = None
+    conc = 0
     if args.kernel == 0:
         from lithofem_amd.tune import tune_local
         tuned = tune_local(g)
         log(f"autotune: best kernel {tuned['best']}: " +
             ", ".join(f"{k}={v:.3f}ms" for k, v in tuned["ms"].items()))
+        if "concurrency" in tuned:
+            conc = tuned["concurrency"]["best"]
+            log(f"autotune: element kernels side by side with the tensor kernel capped at "
+                f"{conc} workgroups (0 = in sequence): " +
+                ", ".join(f"{k}={v:.3f}ms" for k, v in tuned["concurrency"]["ms"].items()))
     else:
         # the captured step launches the handle's default kernel
         g.set_local_kernel(args.kernel)
+    if args.concurrency is not None:
+        conc = args.concurrency
+    # the eager steps time every kernel on its own (element kernels in
+    # sequence); the captured step runs with the chosen concurrency
+    g.set_element_concurrency(0)
 
     def step():
         g.compute_local(gpu.CURLCURL, gpu.MASS_ALL, True, kernel=args.kernel)
@@ -274,18 +293,27 @@ def main() -> int:
     tl_avg, ts_avg, tt_avg, tq_avg = tsum / args.steps
     log(f"eager step {ms_eager:.3f} ms")
     ms_step = ms_eager
+    ms_graph_seq = None
     if args.graph == "on":
-        # the same step as one captured HIP graph per launch
-        g.step_capture(gpu.CURLCURL, gpu.MASS_ALL, True, True)
-        for i in range(args.warmup):
-            g.step_launch(sync=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            g.step_launch(sync=False)
-        torch.cuda.synchronize()       # device-wide: waits for the library's streams too
-        ms_step = (time.perf_counter() - t0) / args.steps * 1e3
-        log(f"graph step {ms_step:.3f} ms")
+        def graph_ms():
+            # the same step as one captured HIP graph per launch
+            g.step_capture(gpu.CURLCURL, gpu.MASS_ALL, True, True)
+            for i in range(args.warmup):
+                g.step_launch(sync=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(args.steps):
+                g.step_launch(sync=False)
+            torch.cuda.synchronize()       # device-wide: waits for the library's streams too
+            return (time.perf_counter() - t0) / args.steps * 1e3
+        if conc > 0:
+            ms_graph_seq = graph_ms()
+            log(f"graph step, element kernels in sequence {ms_graph_seq:.3f} ms")
+            g.set_element_concurrency(conc)
+        ms_step = graph_ms()
+        log(f"graph step {ms_step:.3f} ms" +
+            (f" (element kernels side by side, tensor kernel capped at {conc} workgroups)"
+             if conc > 0 else ""))
     log(f"step {ms_step:.3f} ms (element matrices {tl_avg:.3f} ms = tensor {tt_avg:.3f} + "
         f"quadrature {tq_avg:.3f}; scatter {ts_avg:.3f} ms)")
 
@@ -366,6 +394,11 @@ def main() -> int:
         "detail": {"element_matrices_ms": tl_avg, "scatter_ms": ts_avg,
                    "step_ms_eager": ms_eager, "step_launch": "hip graph" if args.graph == "on"
                    else "eager",
+                   "element_concurrency_grid": conc,
+                   "element_concurrency_ms": ({str(k): v for k, v in
+                                               tuned["concurrency"]["ms"].items()}
+                                              if tuned and "concurrency" in tuned else None),
+                   "step_ms_graph_elements_in_sequence": ms_graph_seq,
                    "kernels": kernels,
                    "elements_tensor_path": nconst, "elements_quadrature": nquad,
                    "element_matrices_quadrature_only_ms": quad_only_ms,

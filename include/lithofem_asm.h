@@ -204,6 +204,19 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel);
  * (0 in mode 0). */
 int lfasm_set_tensor_path(lfasm_handle *h, int mode);
 
+/* Run the two element-matrix kernels of the tensor path side by side (no
+ * reference counterpart).  grid = 0 (default): the reference-tensor kernel,
+ * then the quadrature kernel, on the handle's stream.  grid > 0: the
+ * tensor kernel with at most `grid` workgroups along its element chunks
+ * (each workgroup walks the chunks with that stride) on the handle's
+ * stream, and the quadrature kernel on a second stream of the handle
+ * (default priority), forked and joined by events, so that the HBM-bound
+ * tensor kernel holds only part of each CU and the MFMA-bound quadrature
+ * kernel the rest.  Values are bit-identical to grid = 0 (the same
+ * kernels, disjoint outputs).  The launch autotuner chooses grid on the
+ * handle's mesh.  Drops the captured step. */
+int lfasm_set_element_concurrency(lfasm_handle *h, int grid);
+
 /* Captured assembly step (no reference counterpart; the reference launches
  * ComputeLocal and ScatterEliminate one kernel at a time,
  * solver/lithofem_solve.cpp:789-790).  capture records the whole step --

@@ -258,6 +258,12 @@ struct lfasm_handle {
    bool ev_used[T_N] = {};
    int local_kernel = 0;   // tuned choice used by kernel selector 0
    int *sym_queue = nullptr;  // work counter of the persistent quadrature kernel
+   // concurrent element kernels: the tensor kernel on `stream` with at most
+   // conc_grid workgroups along the element chunks, the quadrature kernel on
+   // stream2 (both default priority), forked and joined by events
+   int conc_grid = 0;
+   hipStream_t stream2 = nullptr;
+   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
    int fail(int code, const char *fmt, ...)
    {
@@ -818,7 +824,8 @@ hipError_t launch_tensor(lfasm_handle *h, const LocalParams &P)
       // padded rows: the column-per-lane kernel with whole-line stores
       // element coefficients first, then the entries
       LF_TRY(launch_tensor_coef(h, T));
-      const dim3 grid((unsigned)((h->nconst + TEN21_EPC - 1) / TEN21_EPC),
+      const int nchunk = (h->nconst + TEN21_EPC - 1) / TEN21_EPC;
+      const dim3 grid((unsigned)(h->conc_grid > 0 ? std::min(nchunk, h->conc_grid) : nchunk),
                       (unsigned)((T.npair * T.lda + 255) / 256));
       hipLaunchKernelGGL(k_local_tensor21, grid, dim3(256), 0, h->stream, T, T.elist,
                          (const double *)h->coef, T.Ae);
@@ -964,6 +971,30 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
       hipLaunchKernelGGL(k_fill_zero, dim3(4096), dim3(BLOCK), 0, h->stream, P.Ae, n);
       LF_TRY(hipGetLastError());
    }
+   else if (split && !fused && h->conc_grid > 0 && h->nquad > 0)
+   {
+      // tensor kernel and quadrature kernel side by side
+      LocalParams Pq = P;
+      Pq.elist = h->qlist;
+      Pq.nout = h->nquad;
+      Pq.out_by_elem = 1;
+      LF_TRY(hipEventRecord(h->ev_fork, h->stream));
+      LF_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+      if (timed) { LF_TRY(ev_begin(h, T_LOCAL_TENSOR)); }
+      LF_TRY(launch_tensor(h, P));
+      if (timed) { LF_TRY(ev_end(h, T_LOCAL_TENSOR)); }
+      // the launchers enqueue on h->stream: point it at the second stream
+      // for the quadrature branch
+      hipStream_t s0 = h->stream;
+      h->stream = h->stream2;
+      hipError_t eq = timed ? ev_begin(h, T_LOCAL_QUAD) : hipSuccess;
+      if (eq == hipSuccess) { eq = launch_local(h, Pq, kernel); }
+      if (eq == hipSuccess && timed) { eq = ev_end(h, T_LOCAL_QUAD); }
+      h->stream = s0;
+      LF_TRY(eq);
+      LF_TRY(hipEventRecord(h->ev_join, h->stream2));
+      LF_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+   }
    else if (split)
    {
       if (timed) { LF_TRY(ev_begin(h, T_LOCAL_TENSOR)); }
@@ -1076,6 +1107,9 @@ int lfasm_create(int device, lfasm_handle **out)
 
    for (int i = 0; i < 2 * T_N; ++i) { LF_HIP(hipEventCreate(&h->ev[i])); }
    LF_HIP(hipMalloc((void **)&h->sym_queue, 256));
+   LF_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+   LF_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+   LF_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
    return LFASM_OK;
 }
 
@@ -1097,6 +1131,9 @@ void lfasm_destroy(lfasm_handle *h)
       if (h->pin_ev[k]) { (void)hipEventDestroy(h->pin_ev[k]); }
    }
    dfree(h->sym_queue);
+   if (h->ev_fork) { (void)hipEventDestroy(h->ev_fork); }
+   if (h->ev_join) { (void)hipEventDestroy(h->ev_join); }
+   if (h->stream2) { (void)hipStreamDestroy(h->stream2); }
    if (h->stream) { (void)hipStreamDestroy(h->stream); }
    delete h;
 }
@@ -1878,6 +1915,15 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel)
       return h->fail(LFASM_E_ARG, "kernel selector %d outside [0, %d)", kernel, 2 + N_MFMA_VARIANTS);
    }
    h->local_kernel = kernel;
+   drop_graph(h);
+   return LFASM_OK;
+}
+
+int lfasm_set_element_concurrency(lfasm_handle *h, int grid)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (grid < 0) { return h->fail(LFASM_E_ARG, "concurrency grid %d < 0", grid); }
+   h->conc_grid = grid;
    drop_graph(h);
    return LFASM_OK;
 }

@@ -302,8 +302,6 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P,
                                                         double2 *__restrict__ Ae)
 {
    const int tid = threadIdx.x;
-   const int c0 = blockIdx.x * TEN21_EPC;
-   const int cnt = min(TEN21_EPC, P.nlist - c0);
    const int nd = P.nd, lda = P.lda;
    // face-transform table in LDS (its entries are picked per lane)
    __shared__ double TT[24];
@@ -332,6 +330,14 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P,
    const int fc = (P.apply_te && col_ok) ? pair_face(l - odd, P.p) : -1;
    const bool st0 = live && k0 < nd, st1 = live && k0 + 1 < nd;
 
+   // chunks of TEN21_EPC list elements, grid-stride over blockIdx.x (a grid
+   // narrower than the chunk count keeps the kernel to a few workgroups per
+   // CU, beside the quadrature kernel)
+   const int nchunk = (P.nlist + TEN21_EPC - 1) / TEN21_EPC;
+   for (int cx = blockIdx.x; cx < nchunk; cx += gridDim.x)
+   {
+   const int c0 = cx * TEN21_EPC;
+   const int cnt = min(TEN21_EPC, P.nlist - c0);
    for (int i = 0; i < cnt; ++i)
    {
       const int e = elist[c0 + i];
@@ -387,6 +393,7 @@ __global__ void __launch_bounds__(256) k_local_tensor21(TensorParams P,
       const ten_dv2 v1 = col_ok ? ten_dv2{re[1], im[1]} : ten_dv2{0.0, 0.0};
       if (st0) { __builtin_nontemporal_store(v0, A + (size_t)k0 * lda + l); }
       if (st1) { __builtin_nontemporal_store(v1, A + (size_t)(k0 + 1) * lda + l); }
+   }
    }
 }
 

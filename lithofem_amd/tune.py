@@ -3,7 +3,10 @@
 Times every launch variant of the quadrature element-matrix kernels (the
 VALU kernel and the MFMA variants: elements per workgroup, waves per EU,
 pipelined or not) on the handle's own mesh with HIP events, and makes the
-fastest one the handle's default.  Results are plain dicts so callers can log them.
+fastest one the handle's default; then, when the reference-tensor kernel
+and the quadrature kernel both run, the workgroup cap under which they run
+side by side (0 = in sequence).  Results are plain dicts so callers can log
+them.
 """
 
 from __future__ import annotations
@@ -33,4 +36,34 @@ def tune_local(g: "gpu.AsmGpuGlobal", reps: int = 3, mask_cc: int = gpu.CURLCURL
         ms[k] = best
     choice = min(ms, key=ms.get)
     g.set_local_kernel(choice)
+    out = {"best": choice, "ms": ms}
+    # both element kernels in use (tensor path with PML elements): also
+    # choose whether and how far they run side by side
+    g.compute_local(mask_cc, mask_mass, True)
+    t = g.timings()
+    if t["local_tensor_ms"] > 0 and t["local_quad_ms"] > 0:
+        out["concurrency"] = tune_concurrency(g, reps, mask_cc, mask_mass)
+    return out
+
+
+# workgroup caps of the tensor kernel tried beside the quadrature kernel
+CONCURRENCY_GRIDS = (0, 64, 96, 128, 160, 224)
+
+
+def tune_concurrency(g: "gpu.AsmGpuGlobal", reps: int = 3, mask_cc: int = gpu.CURLCURL,
+                     mask_mass: int = gpu.MASS_ALL) -> dict:
+    """Times the whole element-matrix phase with the tensor kernel and the
+    quadrature kernel in sequence (0) and side by side with the tensor
+    kernel held to each grid cap, and keeps the fastest."""
+    ms = {}
+    for grid in CONCURRENCY_GRIDS:
+        g.set_element_concurrency(grid)
+        g.compute_local(mask_cc, mask_mass, True)   # warm-up launch
+        best = float("inf")
+        for _ in range(reps):
+            g.compute_local(mask_cc, mask_mass, True)
+            best = min(best, g.timings()["local_ms"])
+        ms[grid] = best
+    choice = min(ms, key=ms.get)
+    g.set_element_concurrency(choice)
     return {"best": choice, "ms": ms}
