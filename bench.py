@@ -179,6 +179,9 @@ def main() -> int:
     ap.add_argument("--concurrency", type=int, default=None,
                     help="workgroup cap of the tensor kernel beside the quadrature kernel in "
                          "the captured step (0 = in sequence; default: the autotuner's choice)")
+    ap.add_argument("--scatter-overlap", type=int, default=None, choices=[0, 1],
+                    help="scatter the tensor-only rows beside the quadrature kernel in the "
+                         "captured step (default: the autotuner's choice)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-variant and adjoint figures of the default run")
     ap.add_argument("--mode", default="assemble",
@@ -268,6 +271,17 @@ def main() -> int:
         g.set_local_kernel(args.kernel)
     if args.concurrency is not None:
         conc = args.concurrency
+    scat = 0
+    tuned_step = None
+    if args.scatter_overlap is not None:
+        scat = args.scatter_overlap
+    elif conc > 0 and args.graph == "on":
+        from lithofem_amd.tune import tune_step
+        tuned_step = tune_step(g, conc)
+        conc, scat = tuned_step["best"]
+        log(f"autotune: captured step (tensor cap, tensor-only rows scattered beside the "
+            f"quadrature kernel) = ({conc}, {scat}): " +
+            ", ".join(f"{k}={v:.3f}ms" for k, v in tuned_step["ms"].items()))
     # the eager steps time every kernel on its own (element kernels in
     # sequence); the captured step runs with the chosen concurrency
     g.set_element_concurrency(0)
@@ -309,10 +323,11 @@ def main() -> int:
         if conc > 0:
             ms_graph_seq = graph_ms()
             log(f"graph step, element kernels in sequence {ms_graph_seq:.3f} ms")
-            g.set_element_concurrency(conc)
+            g.set_element_concurrency(conc, scat)
         ms_step = graph_ms()
         log(f"graph step {ms_step:.3f} ms" +
-            (f" (element kernels side by side, tensor kernel capped at {conc} workgroups)"
+            (f" (element kernels side by side, tensor kernel capped at {conc} workgroups; "
+             f"tensor-only rows scattered beside the quadrature kernel: {scat})"
              if conc > 0 else ""))
     log(f"step {ms_step:.3f} ms (element matrices {tl_avg:.3f} ms = tensor {tt_avg:.3f} + "
         f"quadrature {tq_avg:.3f}; scatter {ts_avg:.3f} ms)")
@@ -395,6 +410,9 @@ def main() -> int:
                    "step_ms_eager": ms_eager, "step_launch": "hip graph" if args.graph == "on"
                    else "eager",
                    "element_concurrency_grid": conc,
+                   "step_scatter_overlap": scat,
+                   "step_overlap_ms": ({f"{k[0]},{k[1]}": v for k, v in tuned_step["ms"].items()}
+                                       if tuned_step else None),
                    "element_concurrency_ms": ({str(k): v for k, v in
                                                tuned["concurrency"]["ms"].items()}
                                               if tuned and "concurrency" in tuned else None),

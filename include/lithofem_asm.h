@@ -213,9 +213,15 @@ int lfasm_set_tensor_path(lfasm_handle *h, int mode);
  * (default priority), forked and joined by events, so that the HBM-bound
  * tensor kernel holds only part of each CU and the MFMA-bound quadrature
  * kernel the rest.  Values are bit-identical to grid = 0 (the same
- * kernels, disjoint outputs).  The launch autotuner chooses grid on the
- * handle's mesh.  Drops the captured step. */
-int lfasm_set_element_concurrency(lfasm_handle *h, int grid);
+ * kernels, disjoint outputs).  scatter_overlap = 1 (with grid > 0) changes
+ * the captured step only (lfasm_step_capture): the CSR rows whose
+ * contributors are all tensor-path elements are scattered on the tensor
+ * branch right after the tensor kernel, while the quadrature kernel still
+ * runs; the other rows are scattered after the join.  Every row is formed
+ * by exactly one scatter, in the same contributor order, so the values are
+ * again bit-identical.  The launch autotuner chooses both on the handle's
+ * mesh.  Drops the captured step. */
+int lfasm_set_element_concurrency(lfasm_handle *h, int grid, int scatter_overlap);
 
 /* Captured assembly step (no reference counterpart; the reference launches
  * ComputeLocal and ScatterEliminate one kernel at a time,

@@ -262,6 +262,13 @@ struct lfasm_handle {
    // conc_grid workgroups along the element chunks, the quadrature kernel on
    // stream2 (both default priority), forked and joined by events
    int conc_grid = 0;
+   // captured step only (scat_overlap = 1): rows whose contributors are all
+   // tensor-path elements (rows_t) are scattered on the tensor branch while
+   // the quadrature kernel runs; the other rows (rows_q) after the join
+   int scat_overlap = 0;
+   int *rows_t = nullptr, *rows_q = nullptr;
+   int nrows_t = 0, nrows_q = 0;
+   bool rows_split = false;
    hipStream_t stream2 = nullptr;
    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
@@ -364,6 +371,9 @@ void release_global(lfasm_handle *h)
    dfree(h->bad); dfree(h->slot); dfree(h->ess); dfree(h->eflag); dfree(h->vals);
    dfree(h->rsoff); dfree(h->rslot);
    dfree(h->emask);
+   dfree(h->rows_t); dfree(h->rows_q);
+   h->nrows_t = h->nrows_q = 0;
+   h->rows_split = false;
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
    dfree(h->rhs); dfree(h->d_deps); dfree(h->d_depsT); dfree(h->vfix); dfree(h->vcopied);
    h->nvfix = 0;
@@ -1021,7 +1031,17 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
 }
 
 // Scatter + elimination of the element matrices into the CSR values.
+hipError_t enqueue_scatter_rows(lfasm_handle *h, int eliminate, bool timed, const int *rows,
+                                int nrows);
+
 hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
+{
+   return enqueue_scatter_rows(h, eliminate, timed, nullptr, h->height);
+}
+
+// Scatter + elimination of the rows `rows` (NULL: all rows).
+hipError_t enqueue_scatter_rows(lfasm_handle *h, int eliminate, bool timed, const int *rows,
+                                int nrows)
 {
    if (h->local_mode == 2)
    {
@@ -1049,24 +1069,96 @@ hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
    S.sld = h->sld;
    S.rsoff = h->rsoff;
    S.rslot = h->rslot;
+   S.rows = rows;
+   S.nrows = nrows;
+   if (nrows <= 0) { return hipSuccess; }
    const size_t shmem = (size_t)4 * h->maxrow * sizeof(double2);
    const bool bytes = h->rslot != nullptr;
    // nd <= 32: two contributors per load instruction
    const bool pack2 = h->nd <= 32;
-   const void *kern = bytes ? (pack2 ? (const void *)k_scatter_rows<true, true>
-                                     : (const void *)k_scatter_rows<true, false>)
-                            : (pack2 ? (const void *)k_scatter_rows<false, true>
-                                     : (const void *)k_scatter_rows<false, false>);
+   const bool list = rows != nullptr;
+   const void *kern =
+      list ? (bytes ? (pack2 ? (const void *)k_scatter_row_list<true, true>
+                             : (const void *)k_scatter_row_list<true, false>)
+                    : (pack2 ? (const void *)k_scatter_row_list<false, true>
+                             : (const void *)k_scatter_row_list<false, false>))
+           : (bytes ? (pack2 ? (const void *)k_scatter_rows<true, true>
+                             : (const void *)k_scatter_rows<true, false>)
+                    : (pack2 ? (const void *)k_scatter_rows<false, true>
+                             : (const void *)k_scatter_rows<false, false>));
    LF_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
    if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
-   const dim3 grid((h->height + 3) / 4);
-   if (bytes && pack2) { hipLaunchKernelGGL((k_scatter_rows<true, true>), grid, dim3(BLOCK), shmem, h->stream, S); }
-   else if (bytes) { hipLaunchKernelGGL((k_scatter_rows<true, false>), grid, dim3(BLOCK), shmem, h->stream, S); }
-   else if (pack2) { hipLaunchKernelGGL((k_scatter_rows<false, true>), grid, dim3(BLOCK), shmem, h->stream, S); }
-   else { hipLaunchKernelGGL((k_scatter_rows<false, false>), grid, dim3(BLOCK), shmem, h->stream, S); }
+   const dim3 grid((unsigned)((nrows + 3) / 4));
+   void *args[] = {&S};
+   LF_TRY(hipLaunchKernel(kern, grid, dim3(BLOCK), args, shmem, h->stream));
    LF_TRY(hipGetLastError());
    if (timed) { LF_TRY(ev_end(h, T_SCATTER)); }
    return hipSuccess;
+}
+
+// Row lists of the overlapped captured step: rows formed only by
+// tensor-path elements and the others, in ascending row order (built once
+// per pattern and element classification, outside any capture).
+hipError_t prepare_row_split(lfasm_handle *h)
+{
+   if (h->rows_split) { return hipSuccess; }
+   std::vector<unsigned char> flag((size_t)h->height);
+   unsigned char *d_flag = nullptr;
+   LF_TRY(hipMalloc((void **)&d_flag, std::max<size_t>(flag.size(), 1)));
+   hipLaunchKernelGGL(k_row_quad_flag, dim3((unsigned)((h->height + 255) / 256)), dim3(256), 0,
+                      h->stream, h->height, h->nd, h->rc_ptr, h->rc_list, h->eclass, d_flag);
+   hipError_t e = hipGetLastError();
+   if (e == hipSuccess)
+   {
+      e = hipMemcpyAsync(flag.data(), d_flag, flag.size(), hipMemcpyDeviceToHost, h->stream);
+   }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   (void)hipFree(d_flag);
+   LF_TRY(e);
+   std::vector<int> rt, rq;
+   for (int r = 0; r < h->height; ++r) { (flag[(size_t)r] ? rq : rt).push_back(r); }
+   h->nrows_t = (int)rt.size();
+   h->nrows_q = (int)rq.size();
+   LF_TRY(upload(h, h->rows_t, rt.data(), rt.size()));
+   LF_TRY(upload(h, h->rows_q, rq.data(), rq.size()));
+   h->rows_split = true;
+   return hipSuccess;
+}
+
+// The overlapped step applies: materialised tensor path with PML elements,
+// both element kernels side by side, a scatter cap, and a mask that the
+// tensor kernel takes (not exactly one of CROSS0 / CROSS1).
+bool step_overlaps(const lfasm_handle *h, const LocalParams &P)
+{
+   const bool one_cross = ((P.mflags >> 1) & 1) != ((P.mflags >> 2) & 1);
+   return h->tensor_mode == 1 && h->nconst > 0 && h->nquad > 0 && h->conc_grid > 0 &&
+          h->scat_overlap && !one_cross && (P.use_cc || P.mflags);
+}
+
+// Captured step with the scatter of the tensor-only rows on the tensor
+// branch: tensor kernel -> scatter of rows_t on the handle's stream,
+// quadrature kernel on the second stream, then (joined) the scatter of
+// rows_q.  Each CSR row is written by exactly one of the two scatters.
+hipError_t enqueue_step_overlap(lfasm_handle *h, const LocalParams &P, int eliminate)
+{
+   h->last_P = P;
+   h->local_mode = 1;
+   LocalParams Pq = P;
+   Pq.elist = h->qlist;
+   Pq.nout = h->nquad;
+   Pq.out_by_elem = 1;
+   LF_TRY(hipEventRecord(h->ev_fork, h->stream));
+   LF_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+   LF_TRY(launch_tensor(h, P));
+   LF_TRY(enqueue_scatter_rows(h, eliminate, false, h->rows_t, h->nrows_t));
+   hipStream_t s0 = h->stream;
+   h->stream = h->stream2;
+   const hipError_t eq = launch_local(h, Pq, 0);
+   h->stream = s0;
+   LF_TRY(eq);
+   LF_TRY(hipEventRecord(h->ev_join, h->stream2));
+   LF_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+   return enqueue_scatter_rows(h, eliminate, false, h->rows_q, h->nrows_q);
 }
 
 }  // namespace
@@ -1919,11 +2011,16 @@ int lfasm_set_local_kernel(lfasm_handle *h, int kernel)
    return LFASM_OK;
 }
 
-int lfasm_set_element_concurrency(lfasm_handle *h, int grid)
+int lfasm_set_element_concurrency(lfasm_handle *h, int grid, int scatter_overlap)
 {
    if (!h) { return LFASM_E_STATE; }
-   if (grid < 0) { return h->fail(LFASM_E_ARG, "concurrency grid %d < 0", grid); }
+   if (grid < 0 || scatter_overlap < 0 || scatter_overlap > 1)
+   {
+      return h->fail(LFASM_E_ARG, "concurrency grid %d / scatter overlap %d out of range", grid,
+                     scatter_overlap);
+   }
    h->conc_grid = grid;
+   h->scat_overlap = scatter_overlap;
    drop_graph(h);
    return LFASM_OK;
 }
@@ -1956,10 +2053,17 @@ int lfasm_step_capture(lfasm_handle *h, int mask_cc, int mask_mass, int apply_te
    h->g_apply_te = apply_te;
    h->g_elim = eliminate;
    const LocalParams P = local_params(h, mask_cc, mask_mass, apply_te, h->Ae);
+   const bool overlap = step_overlaps(h, P);
+   if (overlap) { LF_HIP(prepare_row_split(h)); }
    LF_HIP(hipStreamSynchronize(h->stream));
    LF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-   hipError_t e = enqueue_local(h, P, 0, false);
-   if (e == hipSuccess) { e = enqueue_scatter(h, eliminate, false); }
+   hipError_t e = hipSuccess;
+   if (overlap) { e = enqueue_step_overlap(h, P, eliminate); }
+   else
+   {
+      e = enqueue_local(h, P, 0, false);
+      if (e == hipSuccess) { e = enqueue_scatter(h, eliminate, false); }
+   }
    hipGraph_t g = nullptr;
    const hipError_t e2 = hipStreamEndCapture(h->stream, &g);
    if (e == hipSuccess) { e = e2; }

@@ -968,6 +968,8 @@ struct ScatterParams {
    const int *rsoff;            // per row: start of its slot block in rslot
    const uint8_t *rslot;        // the row's contributors' slot rows as bytes, in
                                 // contributor order (rows of at most 256 entries)
+   const int *rows;             // rows to form (NULL: rows 0 .. nrows-1)
+   int nrows;
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -1166,15 +1168,10 @@ __device__ __forceinline__ void scatter_dispatch(const ScatterParams &S, double2
 constexpr int SCATTER_NP = 4;   // contributor pairs per group of the packed form (nd <= 32)
 
 // PACK2 (nd <= 32): two contributors per load instruction (scatter_group2)
-template <bool BYTE_SLOTS, bool PACK2 = false>
-__global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
+// Row r of the CSR values by one wavefront (buf: its LDS row buffer).
+template <bool BYTE_SLOTS, bool PACK2>
+__device__ __forceinline__ void scatter_row(const ScatterParams &S, double2 *buf, int r, int lane)
 {
-   extern __shared__ double2 rowbuf[];
-   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-   const int lane = threadIdx.x & 63;
-   const int r = blockIdx.x * 4 + wave;
-   if (r >= S.n) { return; }
-   double2 *buf = rowbuf + (size_t)wave * S.maxrow;
    const int base = S.Ip[r];
    const int len = S.Ip[r + 1] - base;
    for (int s = lane; s < len; s += 64) { buf[s] = make_double2(0.0, 0.0); }
@@ -1221,6 +1218,47 @@ __global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
       const double2 t = buf[s];
       __builtin_nontemporal_store(dbl2v{t.x, t.y}, reinterpret_cast<dbl2v *>(S.vals) + base + s);
    }
+}
+
+// Every row, one per wavefront, 4 rows per workgroup.
+template <bool BYTE_SLOTS, bool PACK2 = false>
+__global__ void __launch_bounds__(BLOCK) k_scatter_rows(ScatterParams S)
+{
+   extern __shared__ double2 rowbuf[];
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + wave;
+   if (r >= S.n) { return; }
+   scatter_row<BYTE_SLOTS, PACK2>(S, rowbuf + (size_t)wave * S.maxrow, r, lane);
+}
+
+// The rows of the list S.rows (S.nrows of them), one per wavefront.
+template <bool BYTE_SLOTS, bool PACK2 = false>
+__global__ void __launch_bounds__(BLOCK) k_scatter_row_list(ScatterParams S)
+{
+   extern __shared__ double2 rowbuf[];
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int ri = blockIdx.x * 4 + wave;
+   if (ri >= S.nrows) { return; }
+   scatter_row<BYTE_SLOTS, PACK2>(S, rowbuf + (size_t)wave * S.maxrow, S.rows[ri], lane);
+}
+
+// Rows with at least one contributor outside the constant-coefficient
+// (tensor-path) element class: flag[r] = 1.
+__global__ void k_row_quad_flag(int n, int nd, const int *__restrict__ rc_ptr,
+                                const int *__restrict__ rc_list,
+                                const unsigned char *__restrict__ eclass,
+                                unsigned char *__restrict__ flag)
+{
+   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= n) { return; }
+   unsigned char f = 0;
+   for (int k = rc_ptr[r]; k < rc_ptr[r + 1]; ++k)
+   {
+      if (!eclass[rc_list[k] / nd]) { f = 1; }
+   }
+   flag[r] = f;
 }
 
 // ---- complex CSR SpMV (solver/asm_gpu.inc:481-493) ------------------------

@@ -121,7 +121,7 @@ def lib():
         "lfasm_download_rhs": ([vp, vp, vp], i),
         "lfasm_eps_sensitivity": ([vp, i, vp, vp, vp, i, vp], i),
         "lfasm_set_tensor_path": ([vp, i], i),
-        "lfasm_set_element_concurrency": ([vp, i], i),
+        "lfasm_set_element_concurrency": ([vp, i, i], i),
         "lfasm_step_capture": ([vp, i, i, i, i], i),
         "lfasm_step_launch": ([vp, i], i),
         "lfasm_build_global_device": ([vp, vp], i),
@@ -437,11 +437,13 @@ class AsmGpuGlobal:
             raise AsmGpuError(rc, self.last_error())
         return rc
 
-    def set_element_concurrency(self, grid: int) -> None:
+    def set_element_concurrency(self, grid: int, scatter_overlap: int = 0) -> None:
         """0: the tensor-path kernel, then the quadrature kernel; grid > 0:
         the two side by side, the tensor kernel held to `grid` workgroups
-        along its element chunks (lfasm_set_element_concurrency)."""
-        self._check(lib().lfasm_set_element_concurrency(self._h, int(grid)))
+        along its element chunks; scatter_overlap = 1: in the captured step,
+        the rows formed only by tensor-path elements are scattered beside
+        the quadrature kernel (lfasm_set_element_concurrency)."""
+        self._check(lib().lfasm_set_element_concurrency(self._h, int(grid), int(scatter_overlap)))
 
     def stream(self) -> int:
         return lib().lfasm_stream(self._h) or 0

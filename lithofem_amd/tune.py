@@ -67,3 +67,31 @@ def tune_concurrency(g: "gpu.AsmGpuGlobal", reps: int = 3, mask_cc: int = gpu.CU
     choice = min(ms, key=ms.get)
     g.set_element_concurrency(choice)
     return {"best": choice, "ms": ms}
+
+
+
+
+def tune_step(g: "gpu.AsmGpuGlobal", grid: int, reps: int = 5,
+              mask_cc: int = gpu.CURLCURL, mask_mass: int = gpu.MASS_ALL) -> dict:
+    """Times the captured step (element matrices, scatter, elimination) with
+    the element kernels side by side under `grid`, with and without the
+    rows formed by tensor-path elements only scattered beside the
+    quadrature kernel -- the latter also with wider tensor caps, under which
+    the quadrature kernel runs more beside that scatter than beside the
+    tensor kernel -- and keeps the fastest (grid, overlap) pair."""
+    ms = {}
+    cands = [(grid, 0), (grid, 1)]
+    if grid > 0:
+        cands += [(2 * grid, 1), (4 * grid, 1), (1 << 20, 1)]
+    for gr, sg in cands:
+        g.set_element_concurrency(gr, sg)
+        g.step_capture(mask_cc, mask_mass, True, True)
+        g.step_launch(sync=True)                    # warm-up replay
+        best = float("inf")
+        for _ in range(reps):
+            g.step_launch(sync=True)
+            best = min(best, g.timings()["step_ms"])
+        ms[(gr, sg)] = best
+    choice = min(ms, key=ms.get)
+    g.set_element_concurrency(*choice)
+    return {"best": choice, "ms": ms}

@@ -1,7 +1,9 @@
 """Element kernels side by side (lfasm_set_element_concurrency): the
 reference-tensor kernel on the handle's stream, held to a workgroup cap,
 and the quadrature kernel on a second stream give exactly the values of
-the two kernels in sequence, eager and as captured-step replays."""
+the two kernels in sequence, eager and as captured-step replays; so does
+the captured step that scatters the rows formed only by tensor-path
+elements beside the quadrature kernel."""
 
 from __future__ import annotations
 
@@ -14,8 +16,8 @@ from lithofem_amd import gpu
 pytestmark = pytest.mark.gpu
 
 
-def _assemble(g, conc, graph):
-    g.set_element_concurrency(conc)
+def _assemble(g, conc, graph, scat=0):
+    g.set_element_concurrency(conc, scat)
     if graph:
         g.step_capture(gpu.CURLCURL, gpu.MASS_ALL, True, True)
         outs = []
@@ -44,6 +46,11 @@ def test_concurrent_element_kernels_match_sequence(conc):
         ref = _assemble(g, 0, False)
         for graph in (False, True):
             got = _assemble(g, conc, graph)
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        # the captured step with the tensor-only rows scattered beside the
+        # quadrature kernel (two row lists, every row formed once)
+        for scat in (1,):
+            got = _assemble(g, conc, True, scat)
             assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
         # back to the sequence, captured
         got = _assemble(g, 0, True)
