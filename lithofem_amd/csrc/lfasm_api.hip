@@ -584,14 +584,6 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
    case 3 * 16 + 5: return sym_applies(P) ? launch_mfma_sym<1, 3, 1>(h, P) : launch_mfma_pipe<3, 1, 2>(h, P);
    case 3 * 16 + 6: return sym_applies(P) ? launch_mfma_sym<1, 3, 1, true>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
    case 3 * 16 + 7: return sym_applies(P) ? launch_mfma_sym<1, 3, 2, true>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 5 * 16 + 0: return launch_mfma_pipe<5, 1, 2>(h, P);
-   case 5 * 16 + 1: return launch_mfma_pipe<5, 1, 1>(h, P);
-   case 5 * 16 + 2: return launch_mfma<5, 1, 1, 2>(h, P);
-   case 5 * 16 + 3: return launch_mfma_pipe<5, 2, 2>(h, P);
-   case 5 * 16 + 4: return launch_mfma_pipe<5, 1, 3>(h, P);
-   case 5 * 16 + 5: return launch_mfma<5, 1, 2, 2>(h, P);
-   case 5 * 16 + 6: return launch_mfma_pipe<5, 1, 2>(h, P);
-   case 5 * 16 + 7: return launch_mfma_pipe<5, 1, 2>(h, P);
    case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
    case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
    case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
@@ -611,8 +603,8 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
 hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel)
 {
    const int nd = P.nd;
-   // 16-wide column tiles: p = 1..4 (nd 6, 20, 45, 70) take 1, 2, 3, 5
-   const int nt = nd <= 16 ? 1 : nd <= 32 ? 2 : nd <= 48 ? 3 : nd <= 80 ? 5 : 6;
+   // 16-wide column tiles: p = 1..4 (nd 6, 20, 45, 84) take 1, 2, 3, 6
+   const int nt = nd <= 16 ? 1 : nd <= 32 ? 2 : nd <= 48 ? 3 : 6;
    if (kernel >= 2 && kernel < 2 + N_MFMA_VARIANTS) { return launch_mfma_variant(h, P, nt, kernel - 2); }
    if (kernel == 0 && h->local_kernel != 0) { return launch_local(h, P, h->local_kernel); }
    if (kernel == 0 && nd > 16) { return launch_mfma_variant(h, P, nt, 0); }
