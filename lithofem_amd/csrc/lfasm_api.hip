@@ -270,7 +270,7 @@ struct lfasm_handle {
    int nrows_t = 0, nrows_q = 0;
    bool rows_split = false;
    hipStream_t stream2 = nullptr;
-   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_tensor = nullptr;
 
    int fail(int code, const char *fmt, ...)
    {
@@ -1127,10 +1127,12 @@ bool step_overlaps(const lfasm_handle *h, const LocalParams &P)
           h->scat_overlap && !one_cross && (P.use_cc || P.mflags);
 }
 
-// Captured step with the scatter of the tensor-only rows on the tensor
-// branch: tensor kernel -> scatter of rows_t on the handle's stream,
-// quadrature kernel on the second stream, then (joined) the scatter of
-// rows_q.  Each CSR row is written by exactly one of the two scatters.
+// Captured step with the scatters beside the element kernels: the tensor
+// kernel, then the scatter of rows_t (rows formed only by tensor-path
+// elements) on the handle's stream; the quadrature kernel on the second
+// stream, then -- once the tensor kernel is done too -- the scatter of
+// rows_q there, beside the rows_t scatter; joined at the end.  Each CSR row
+// is written by exactly one of the two scatters.
 hipError_t enqueue_step_overlap(lfasm_handle *h, const LocalParams &P, int eliminate)
 {
    h->last_P = P;
@@ -1142,15 +1144,18 @@ hipError_t enqueue_step_overlap(lfasm_handle *h, const LocalParams &P, int elimi
    LF_TRY(hipEventRecord(h->ev_fork, h->stream));
    LF_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
    LF_TRY(launch_tensor(h, P));
+   LF_TRY(hipEventRecord(h->ev_tensor, h->stream));
    LF_TRY(enqueue_scatter_rows(h, eliminate, false, h->rows_t, h->nrows_t));
+   // the launchers enqueue on h->stream: point it at the second stream
    hipStream_t s0 = h->stream;
    h->stream = h->stream2;
-   const hipError_t eq = launch_local(h, Pq, 0);
+   hipError_t e = launch_local(h, Pq, 0);
+   if (e == hipSuccess) { e = hipStreamWaitEvent(h->stream2, h->ev_tensor, 0); }
+   if (e == hipSuccess) { e = enqueue_scatter_rows(h, eliminate, false, h->rows_q, h->nrows_q); }
    h->stream = s0;
-   LF_TRY(eq);
+   LF_TRY(e);
    LF_TRY(hipEventRecord(h->ev_join, h->stream2));
-   LF_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
-   return enqueue_scatter_rows(h, eliminate, false, h->rows_q, h->nrows_q);
+   return hipStreamWaitEvent(h->stream, h->ev_join, 0);
 }
 
 }  // namespace
@@ -1194,6 +1199,7 @@ int lfasm_create(int device, lfasm_handle **out)
    LF_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
    LF_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
    LF_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+   LF_HIP(hipEventCreateWithFlags(&h->ev_tensor, hipEventDisableTiming));
    return LFASM_OK;
 }
 
@@ -1217,6 +1223,7 @@ void lfasm_destroy(lfasm_handle *h)
    dfree(h->sym_queue);
    if (h->ev_fork) { (void)hipEventDestroy(h->ev_fork); }
    if (h->ev_join) { (void)hipEventDestroy(h->ev_join); }
+   if (h->ev_tensor) { (void)hipEventDestroy(h->ev_tensor); }
    if (h->stream2) { (void)hipStreamDestroy(h->stream2); }
    if (h->stream) { (void)hipStreamDestroy(h->stream); }
    delete h;
