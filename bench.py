@@ -500,10 +500,14 @@ def _quad_kernel_name(selector: int, nd: int) -> str:
     with all five integrators (the launch table of lfasm_api.hip)."""
     if not 32 < nd <= 48:
         return f"selector {selector}"
-    return {1: "k_local_generic<4, 512> + k_te", 2: "k_local_mfma_sym<1, 3>",
-            3: "k_local_mfma_sym<2, 3, true>", 4: "k_local_mfma_pipe<3, 1, 4>",
-            5: "k_local_mfma<3, 1, 1, 2>", 6: "k_local_mfma_sym<2, 3>",
-            7: "k_local_mfma_sym<1, 3, true>"}.get(selector, f"selector {selector}")
+    # k_local_mfma_sym<staging buffers, waves per EU, priority mode, persistent>
+    # (after its setup-record kernel k_sym_prep); priority 1: raised for
+    # the MFMAs, 2: also for the epilogue
+    return {1: "k_local_generic<4, 512> + k_te", 2: "k_local_mfma_sym<1, 3, 0, false>",
+            3: "k_local_mfma_sym<2, 3, 1, false>", 4: "k_local_mfma_pipe<3, 1, 4>",
+            5: "k_local_mfma<3, 1, 1, 2>", 6: "k_local_mfma_sym<1, 3, 2, false>",
+            7: "k_local_mfma_sym<1, 3, 1, false>", 8: "k_local_mfma_sym<1, 3, 1, true>",
+            9: "k_local_mfma_sym<1, 3, 2, true>"}.get(selector, f"selector {selector}")
 
 
 def _measured_traffic() -> dict:
