@@ -447,16 +447,16 @@ LocalParams local_params(lfasm_handle *h, int mask_cc, int mask_mass, int apply_
 
 // Setup records of the general MFMA kernels (k_elem_prep), one wavefront
 // per list entry; each record sits in its entry's own element-matrix slot.
-hipError_t launch_elem_prep(lfasm_handle *h, const LocalParams &P)
+hipError_t launch_elem_prep(lfasm_handle *h, const LocalParams &P, hipStream_t st)
 {
    const int nqt = P.rc.nq + P.rm.nq;
    if ((size_t)(EC_SIZE + 4 * nqt) > 2 * (size_t)P.nd * P.lda) { return hipErrorInvalidValue; }
-   hipLaunchKernelGGL(k_elem_prep, dim3((unsigned)P.nout), dim3(64), 0, h->stream, P, nqt);
+   hipLaunchKernelGGL(k_elem_prep, dim3((unsigned)P.nout), dim3(64), 0, st, P, nqt);
    return hipGetLastError();
 }
 
 template <int NT, int RT, int EPB, int WPEU>
-hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P)
+hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P, hipStream_t st)
 {
    constexpr int NP = 16 * NT;
    constexpr int WPE = NT / RT;
@@ -466,15 +466,15 @@ hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma<NT, RT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   LF_TRY(launch_elem_prep(h, P));
+   LF_TRY(launch_elem_prep(h, P, st));
    const int grid = (P.nout + EPB - 1) / EPB;
    hipLaunchKernelGGL((k_local_mfma<NT, RT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
-                      h->stream, P, nqt);
+                      st, P, nqt);
    return hipGetLastError();
 }
 
 template <int NACC, int NTHR>
-hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
+hipError_t launch_generic(lfasm_handle *h, const LocalParams &P, hipStream_t st)
 {
    const int nqt = P.rc.nq + P.rm.nq;
    const int NP = P.nd;
@@ -482,17 +482,17 @@ hipError_t launch_generic(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_generic<NACC, NTHR>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   hipLaunchKernelGGL((k_local_generic<NACC, NTHR>), dim3(P.nout), dim3(NTHR), shmem, h->stream,
+   hipLaunchKernelGGL((k_local_generic<NACC, NTHR>), dim3(P.nout), dim3(NTHR), shmem, st,
                       P, nqt, NP);
    e = hipGetLastError();
    if (e != hipSuccess || !P.apply_te) { return e; }
-   hipLaunchKernelGGL(k_te, dim3(P.nout), dim3(64), 0, h->stream, P.nd, P.lda, P.p, P.fo,
+   hipLaunchKernelGGL(k_te, dim3(P.nout), dim3(64), 0, st, P.nd, P.lda, P.p, P.fo,
                       P.elist, P.out_by_elem, h->TInv_dev, P.Ae);
    return hipGetLastError();
 }
 
 template <int NT, int EPB, int WPEU>
-hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
+hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P, hipStream_t st)
 {
    constexpr int NP = 16 * NT;
    constexpr int THREADS = 64 * NT * EPB;
@@ -502,10 +502,10 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_pipe<NT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
-   LF_TRY(launch_elem_prep(h, P));
+   LF_TRY(launch_elem_prep(h, P, st));
    const int grid = (P.nout + EPB - 1) / EPB;
    hipLaunchKernelGGL((k_local_mfma_pipe<NT, EPB, WPEU>), dim3(grid), dim3(THREADS), shmem,
-                      h->stream, P, nqt);
+                      st, P, nqt);
    return hipGetLastError();
 }
 
@@ -513,7 +513,7 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P)
 // or both off), after its setup-record kernel.  PERSIST: as many workgroups
 // as fit on the device at once, fed from the handle's work counter.
 template <int NBUF, int WPEU, int PRIO = 0, bool PERSIST = false>
-hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
+hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P, hipStream_t st)
 {
    const int nqt = P.rc.nq + P.rm.nq;
    // each entry's setup record sits in its own element-matrix slot
@@ -521,7 +521,7 @@ hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
    // the persistent form prefetches a record in SYM_PF registers per thread
    if (PERSIST && sym_rec_size(nqt) > SYM_PF * SYM_THREADS)
    {
-      return launch_mfma_sym<NBUF, WPEU, PRIO, false>(h, P);
+      return launch_mfma_sym<NBUF, WPEU, PRIO, false>(h, P, st);
    }
    const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + SYM_FCS + NBUF * SYM_STG) * sizeof(double);
    auto kern = k_local_mfma_sym<NBUF, WPEU, PRIO, PERSIST>;
@@ -537,11 +537,11 @@ hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P)
       if (e != hipSuccess) { return e; }
       grid = std::max(1, std::min(P.nout, std::max(1, per_cu) * h->cus));
    }
-   hipLaunchKernelGGL(k_sym_prep, dim3((unsigned)P.nout), dim3(64), 0, h->stream, P, nqt,
+   hipLaunchKernelGGL(k_sym_prep, dim3((unsigned)P.nout), dim3(64), 0, st, P, nqt,
                       PERSIST ? h->sym_queue : nullptr);
    e = hipGetLastError();
    if (e != hipSuccess) { return e; }
-   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(SYM_THREADS), shmem, h->stream, P, nqt,
+   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(SYM_THREADS), shmem, st, P, nqt,
                       h->sym_queue);
    return hipGetLastError();
 }
@@ -556,42 +556,43 @@ bool sym_applies(const LocalParams &P)
 // default; the launch autotuner (lithofem_amd.tune) times all of them.
 constexpr int N_MFMA_VARIANTS = LFASM_N_MFMA_VARIANTS;
 
-hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, int v)
+hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, int v,
+                               hipStream_t st)
 {
    switch (nt * 16 + v)
    {
-   case 1 * 16 + 0: return launch_mfma_pipe<1, 4, 2>(h, P);
-   case 1 * 16 + 1: return launch_mfma_pipe<1, 2, 2>(h, P);
-   case 1 * 16 + 2: return launch_mfma<1, 1, 4, 2>(h, P);
-   case 1 * 16 + 3: return launch_mfma_pipe<1, 4, 4>(h, P);
-   case 1 * 16 + 4: return launch_mfma_pipe<1, 4, 2>(h, P);
-   case 1 * 16 + 5: return launch_mfma_pipe<1, 2, 2>(h, P);
-   case 1 * 16 + 6: return launch_mfma_pipe<1, 4, 2>(h, P);
-   case 1 * 16 + 7: return launch_mfma_pipe<1, 4, 2>(h, P);
-   case 2 * 16 + 0: return launch_mfma<2, 2, 4, 2>(h, P);
-   case 2 * 16 + 1: return launch_mfma_pipe<2, 1, 2>(h, P);
-   case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P);
-   case 2 * 16 + 3: return launch_mfma_pipe<2, 2, 2>(h, P);
-   case 2 * 16 + 4: return launch_mfma_pipe<2, 2, 2>(h, P);
-   case 2 * 16 + 5: return launch_mfma_pipe<2, 1, 2>(h, P);
-   case 2 * 16 + 6: return launch_mfma<2, 2, 4, 2>(h, P);
-   case 2 * 16 + 7: return launch_mfma<2, 2, 4, 2>(h, P);
-   case 3 * 16 + 0: return sym_applies(P) ? launch_mfma_sym<1, 3>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<2, 3, 1>(h, P) : launch_mfma_pipe<3, 2, 2>(h, P);
-   case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P);
-   case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<1, 3, 2>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 5: return sym_applies(P) ? launch_mfma_sym<1, 3, 1>(h, P) : launch_mfma_pipe<3, 1, 2>(h, P);
-   case 3 * 16 + 6: return sym_applies(P) ? launch_mfma_sym<1, 3, 1, true>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 3 * 16 + 7: return sym_applies(P) ? launch_mfma_sym<1, 3, 2, true>(h, P) : launch_mfma_pipe<3, 1, 4>(h, P);
-   case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P);
-   case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P);
-   case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P);
-   case 6 * 16 + 3: return launch_mfma_pipe<6, 1, 2>(h, P);
-   case 6 * 16 + 4: return launch_mfma_pipe<6, 1, 2>(h, P);
-   case 6 * 16 + 5: return launch_mfma_pipe<6, 1, 1>(h, P);
-   case 6 * 16 + 6: return launch_mfma_pipe<6, 1, 2>(h, P);
-   case 6 * 16 + 7: return launch_mfma_pipe<6, 1, 2>(h, P);
+   case 1 * 16 + 0: return launch_mfma_pipe<1, 4, 2>(h, P, st);
+   case 1 * 16 + 1: return launch_mfma_pipe<1, 2, 2>(h, P, st);
+   case 1 * 16 + 2: return launch_mfma<1, 1, 4, 2>(h, P, st);
+   case 1 * 16 + 3: return launch_mfma_pipe<1, 4, 4>(h, P, st);
+   case 1 * 16 + 4: return launch_mfma_pipe<1, 4, 2>(h, P, st);
+   case 1 * 16 + 5: return launch_mfma_pipe<1, 2, 2>(h, P, st);
+   case 1 * 16 + 6: return launch_mfma_pipe<1, 4, 2>(h, P, st);
+   case 1 * 16 + 7: return launch_mfma_pipe<1, 4, 2>(h, P, st);
+   case 2 * 16 + 0: return launch_mfma<2, 2, 4, 2>(h, P, st);
+   case 2 * 16 + 1: return launch_mfma_pipe<2, 1, 2>(h, P, st);
+   case 2 * 16 + 2: return launch_mfma<2, 1, 2, 2>(h, P, st);
+   case 2 * 16 + 3: return launch_mfma_pipe<2, 2, 2>(h, P, st);
+   case 2 * 16 + 4: return launch_mfma_pipe<2, 2, 2>(h, P, st);
+   case 2 * 16 + 5: return launch_mfma_pipe<2, 1, 2>(h, P, st);
+   case 2 * 16 + 6: return launch_mfma<2, 2, 4, 2>(h, P, st);
+   case 2 * 16 + 7: return launch_mfma<2, 2, 4, 2>(h, P, st);
+   case 3 * 16 + 0: return sym_applies(P) ? launch_mfma_sym<1, 3>(h, P, st) : launch_mfma_pipe<3, 1, 4>(h, P, st);
+   case 3 * 16 + 1: return sym_applies(P) ? launch_mfma_sym<2, 3, 1>(h, P, st) : launch_mfma_pipe<3, 2, 2>(h, P, st);
+   case 3 * 16 + 2: return launch_mfma_pipe<3, 1, 4>(h, P, st);
+   case 3 * 16 + 3: return launch_mfma<3, 1, 1, 2>(h, P, st);
+   case 3 * 16 + 4: return sym_applies(P) ? launch_mfma_sym<1, 3, 2>(h, P, st) : launch_mfma_pipe<3, 1, 4>(h, P, st);
+   case 3 * 16 + 5: return sym_applies(P) ? launch_mfma_sym<1, 3, 1>(h, P, st) : launch_mfma_pipe<3, 1, 2>(h, P, st);
+   case 3 * 16 + 6: return sym_applies(P) ? launch_mfma_sym<1, 3, 1, true>(h, P, st) : launch_mfma_pipe<3, 1, 4>(h, P, st);
+   case 3 * 16 + 7: return sym_applies(P) ? launch_mfma_sym<1, 3, 2, true>(h, P, st) : launch_mfma_pipe<3, 1, 4>(h, P, st);
+   case 6 * 16 + 0: return launch_mfma_pipe<6, 1, 2>(h, P, st);
+   case 6 * 16 + 1: return launch_mfma_pipe<6, 1, 1>(h, P, st);
+   case 6 * 16 + 2: return launch_mfma<6, 1, 1, 2>(h, P, st);
+   case 6 * 16 + 3: return launch_mfma_pipe<6, 1, 2>(h, P, st);
+   case 6 * 16 + 4: return launch_mfma_pipe<6, 1, 2>(h, P, st);
+   case 6 * 16 + 5: return launch_mfma_pipe<6, 1, 1>(h, P, st);
+   case 6 * 16 + 6: return launch_mfma_pipe<6, 1, 2>(h, P, st);
+   case 6 * 16 + 7: return launch_mfma_pipe<6, 1, 2>(h, P, st);
    default: return hipErrorInvalidValue;
    }
 }
@@ -600,19 +601,19 @@ hipError_t launch_mfma_variant(lfasm_handle *h, const LocalParams &P, int nt, in
 // MFMA variant 0 for nd > 16 -- at p = 2 on 57,660 PML elements 0.61 ms
 // against 1.05 ms for the VALU kernel and its separate face transform --
 // and the VALU kernel for nd <= 16), 1 = VALU kernel, 2 + v = MFMA variant v.
-hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel)
+hipError_t launch_local(lfasm_handle *h, const LocalParams &P, int kernel, hipStream_t st)
 {
    const int nd = P.nd;
    // 16-wide column tiles: p = 1..4 (nd 6, 20, 45, 84) take 1, 2, 3, 6
    const int nt = nd <= 16 ? 1 : nd <= 32 ? 2 : nd <= 48 ? 3 : 6;
-   if (kernel >= 2 && kernel < 2 + N_MFMA_VARIANTS) { return launch_mfma_variant(h, P, nt, kernel - 2); }
-   if (kernel == 0 && h->local_kernel != 0) { return launch_local(h, P, h->local_kernel); }
-   if (kernel == 0 && nd > 16) { return launch_mfma_variant(h, P, nt, 0); }
+   if (kernel >= 2 && kernel < 2 + N_MFMA_VARIANTS) { return launch_mfma_variant(h, P, nt, kernel - 2, st); }
+   if (kernel == 0 && h->local_kernel != 0) { return launch_local(h, P, h->local_kernel, st); }
+   if (kernel == 0 && nd > 16) { return launch_mfma_variant(h, P, nt, 0, st); }
    const int n2 = nd * nd;
-   if (n2 <= 256) { return launch_generic<1, 256>(h, P); }
-   if (n2 <= 512) { return launch_generic<2, 256>(h, P); }
-   if (n2 <= 2048) { return launch_generic<4, 512>(h, P); }
-   if (n2 <= 8192) { return launch_generic<8, 1024>(h, P); }
+   if (n2 <= 256) { return launch_generic<1, 256>(h, P, st); }
+   if (n2 <= 512) { return launch_generic<2, 256>(h, P, st); }
+   if (n2 <= 2048) { return launch_generic<4, 512>(h, P, st); }
+   if (n2 <= 8192) { return launch_generic<8, 1024>(h, P, st); }
    return hipErrorInvalidValue;
 }
 
@@ -935,13 +936,16 @@ int download_unpad(lfasm_handle *h, const double2 *dev, size_t rows, int nd, int
    return LFASM_OK;
 }
 
-hipError_t ev_begin(lfasm_handle *h, int t)
+hipError_t ev_begin(lfasm_handle *h, int t, hipStream_t st = nullptr)
 {
    h->ev_used[t] = true;
-   return hipEventRecord(h->ev[2 * t], h->stream);
+   return hipEventRecord(h->ev[2 * t], st ? st : h->stream);
 }
 
-hipError_t ev_end(lfasm_handle *h, int t) { return hipEventRecord(h->ev[2 * t + 1], h->stream); }
+hipError_t ev_end(lfasm_handle *h, int t, hipStream_t st = nullptr)
+{
+   return hipEventRecord(h->ev[2 * t + 1], st ? st : h->stream);
+}
 
 
 // Element matrices of the whole mesh, enqueued on h->stream without a
@@ -985,15 +989,9 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
       if (timed) { LF_TRY(ev_begin(h, T_LOCAL_TENSOR)); }
       LF_TRY(launch_tensor(h, P));
       if (timed) { LF_TRY(ev_end(h, T_LOCAL_TENSOR)); }
-      // the launchers enqueue on h->stream: point it at the second stream
-      // for the quadrature branch
-      hipStream_t s0 = h->stream;
-      h->stream = h->stream2;
-      hipError_t eq = timed ? ev_begin(h, T_LOCAL_QUAD) : hipSuccess;
-      if (eq == hipSuccess) { eq = launch_local(h, Pq, kernel); }
-      if (eq == hipSuccess && timed) { eq = ev_end(h, T_LOCAL_QUAD); }
-      h->stream = s0;
-      LF_TRY(eq);
+      if (timed) { LF_TRY(ev_begin(h, T_LOCAL_QUAD, h->stream2)); }
+      LF_TRY(launch_local(h, Pq, kernel, h->stream2));
+      if (timed) { LF_TRY(ev_end(h, T_LOCAL_QUAD, h->stream2)); }
       LF_TRY(hipEventRecord(h->ev_join, h->stream2));
       LF_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
    }
@@ -1010,13 +1008,13 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
          Pq.nout = h->nquad;
          Pq.out_by_elem = 1;
          if (timed) { LF_TRY(ev_begin(h, T_LOCAL_QUAD)); }
-         LF_TRY(launch_local(h, Pq, kernel));
+         LF_TRY(launch_local(h, Pq, kernel, h->stream));
          if (timed) { LF_TRY(ev_end(h, T_LOCAL_QUAD)); }
       }
    }
    else
    {
-      LF_TRY(launch_local(h, P, kernel));
+      LF_TRY(launch_local(h, P, kernel, h->stream));
    }
    if (timed) { LF_TRY(ev_end(h, T_LOCAL)); }
    return hipSuccess;
@@ -1024,16 +1022,16 @@ hipError_t enqueue_local(lfasm_handle *h, const LocalParams &P, int kernel, bool
 
 // Scatter + elimination of the element matrices into the CSR values.
 hipError_t enqueue_scatter_rows(lfasm_handle *h, int eliminate, bool timed, const int *rows,
-                                int nrows);
+                                int nrows, hipStream_t st);
 
 hipError_t enqueue_scatter(lfasm_handle *h, int eliminate, bool timed)
 {
-   return enqueue_scatter_rows(h, eliminate, timed, nullptr, h->height);
+   return enqueue_scatter_rows(h, eliminate, timed, nullptr, h->height, h->stream);
 }
 
-// Scatter + elimination of the rows `rows` (NULL: all rows).
+// Scatter + elimination of the rows `rows` (NULL: all rows) on stream st.
 hipError_t enqueue_scatter_rows(lfasm_handle *h, int eliminate, bool timed, const int *rows,
-                                int nrows)
+                                int nrows, hipStream_t st)
 {
    if (h->local_mode == 2)
    {
@@ -1079,12 +1077,12 @@ hipError_t enqueue_scatter_rows(lfasm_handle *h, int eliminate, bool timed, cons
                     : (pack2 ? (const void *)k_scatter_rows<false, true>
                              : (const void *)k_scatter_rows<false, false>));
    LF_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-   if (timed) { LF_TRY(ev_begin(h, T_SCATTER)); }
+   if (timed) { LF_TRY(ev_begin(h, T_SCATTER, st)); }
    const dim3 grid((unsigned)((nrows + 3) / 4));
    void *args[] = {&S};
-   LF_TRY(hipLaunchKernel(kern, grid, dim3(BLOCK), args, shmem, h->stream));
+   LF_TRY(hipLaunchKernel(kern, grid, dim3(BLOCK), args, shmem, st));
    LF_TRY(hipGetLastError());
-   if (timed) { LF_TRY(ev_end(h, T_SCATTER)); }
+   if (timed) { LF_TRY(ev_end(h, T_SCATTER, st)); }
    return hipSuccess;
 }
 
@@ -1145,15 +1143,10 @@ hipError_t enqueue_step_overlap(lfasm_handle *h, const LocalParams &P, int elimi
    LF_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
    LF_TRY(launch_tensor(h, P));
    LF_TRY(hipEventRecord(h->ev_tensor, h->stream));
-   LF_TRY(enqueue_scatter_rows(h, eliminate, false, h->rows_t, h->nrows_t));
-   // the launchers enqueue on h->stream: point it at the second stream
-   hipStream_t s0 = h->stream;
-   h->stream = h->stream2;
-   hipError_t e = launch_local(h, Pq, 0);
-   if (e == hipSuccess) { e = hipStreamWaitEvent(h->stream2, h->ev_tensor, 0); }
-   if (e == hipSuccess) { e = enqueue_scatter_rows(h, eliminate, false, h->rows_q, h->nrows_q); }
-   h->stream = s0;
-   LF_TRY(e);
+   LF_TRY(enqueue_scatter_rows(h, eliminate, false, h->rows_t, h->nrows_t, h->stream));
+   LF_TRY(launch_local(h, Pq, 0, h->stream2));
+   LF_TRY(hipStreamWaitEvent(h->stream2, h->ev_tensor, 0));
+   LF_TRY(enqueue_scatter_rows(h, eliminate, false, h->rows_q, h->nrows_q, h->stream2));
    LF_TRY(hipEventRecord(h->ev_join, h->stream2));
    return hipStreamWaitEvent(h->stream, h->ev_join, 0);
 }
@@ -2322,7 +2315,7 @@ int variants_mhat(lfasm_handle *h)
    }
    else
    {
-      LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1));
+      LF_HIP(launch_local(h, P, h->nd >= 45 ? 2 : 1, h->stream));
    }
    LF_HIP(hipStreamSynchronize(h->stream));
    return LFASM_OK;
