@@ -298,6 +298,16 @@ int lfasm_timings(const lfasm_handle *h, double *ms, int n);
  * interleaved complex; writes nvar value arrays on the device. */
 int lfasm_variants_prepare(lfasm_handle *h, int nparam, const int *elems);
 int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int eliminate);
+/* Variant kernel form (no reference counterpart): 0 = automatic (the
+ * accumulating form when nd <= 64, at most 16 parameter contributors per
+ * row and rows of at most 256 entries; else the register form when nd <= 64
+ * and at most 16 contributors; else the general form), 1 = accumulating
+ * (base row in registers, contributions added with LDS atomics),
+ * 2 = register (contributor entries in registers, read-modify-write of the
+ * row buffer), 3 = general.  Every form sums each entry as base + the
+ * contributions in ascending element order.  assemble fails with
+ * LFASM_E_UNSUPPORTED when a forced form does not fit the mesh. */
+int lfasm_set_variant_kernel(lfasm_handle *h, int kernel);
 int lfasm_variant_values(lfasm_handle *h, int v, void **dev_vals);
 int lfasm_download_variant(lfasm_handle *h, int v, double *re, double *im);
 

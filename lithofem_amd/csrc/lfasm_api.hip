@@ -233,6 +233,7 @@ struct lfasm_handle {
    std::vector<int> pelem_host;
    std::vector<int> dof_host;         // element dof table (host copy)
    int pmaxc = 0;                     // most parameter contributors of one row
+   int variant_kernel = 0;            // 0 automatic, 1 accumulating, 2 register, 3 general
    int2 *vfix = nullptr;              // chunks of long runs of rows without parameter contributors
    int nvfix = 0;
    unsigned char *vcopied = nullptr;  // per row: inside such a run
@@ -2017,6 +2018,14 @@ int lfasm_set_element_concurrency(lfasm_handle *h, int grid, int scatter_overlap
    return LFASM_OK;
 }
 
+int lfasm_set_variant_kernel(lfasm_handle *h, int kernel)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (kernel < 0 || kernel > 3) { return h->fail(LFASM_E_ARG, "variant kernel %d out of range", kernel); }
+   h->variant_kernel = kernel;
+   return LFASM_OK;
+}
+
 int lfasm_set_tensor_path(lfasm_handle *h, int mode)
 {
    if (!h || !h->inited) { return h ? h->fail(LFASM_E_STATE, "init first") : LFASM_E_STATE; }
@@ -2457,16 +2466,30 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    V.eliminate = eliminate ? 1 : 0;
    V.maxrow = h->maxrow;
    V.vcopied = h->vcopied;
-   const size_t shmem = (size_t)4 * 2 * h->maxrow * sizeof(double2);
-   hipError_t e = hipFuncSetAttribute((const void *)k_variant_rows,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-   if (e == hipSuccess)
+   // kernel form: the accumulating one for rows of at most 256 entries, the
+   // register one for longer rows, the general one for nd > 64 or more than
+   // VMAXC contributors in a row
+   const bool reg_ok = h->nd <= 64 && h->pmaxc <= VMAXC;
+   const bool acc_ok = reg_ok && h->maxrow <= 256;
+   int vk = h->variant_kernel;
+   if (vk == 0) { vk = acc_ok ? 1 : reg_ok ? 2 : 3; }
+   if ((vk == 1 && !acc_ok) || (vk == 2 && !reg_ok))
    {
-      e = hipFuncSetAttribute((const void *)k_variant_rows_reg,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      return h->fail(LFASM_E_UNSUPPORTED, "variant kernel %d: nd %d, %d contributors, row length %d", vk,
+                     h->nd, h->pmaxc, h->maxrow);
    }
+   const size_t shmem = vk == 1 ? (size_t)4 * (h->maxrow + 64) * sizeof(double2)
+                                : (size_t)4 * 2 * h->maxrow * sizeof(double2);
+   const int ns = (h->maxrow + 63) / 64;
+   const void *kfun = vk == 3 ? (const void *)k_variant_rows
+                      : vk == 2 ? (const void *)k_variant_rows_reg
+                      : ns <= 1 ? (const void *)k_variant_rows_acc<1>
+                      : ns == 2 ? (const void *)k_variant_rows_acc<2>
+                      : ns == 3 ? (const void *)k_variant_rows_acc<3>
+                                : (const void *)k_variant_rows_acc<4>;
+   hipError_t e = hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
-   if (e == hipSuccess && h->nd <= 64 && h->pmaxc <= VMAXC)
+   if (e == hipSuccess && vk != 3)
    {
       const dim3 tg((unsigned)((h->nparam + 31) / 32), (unsigned)((nvar + 31) / 32));
       hipLaunchKernelGGL(k_transpose_deps, tg, dim3(256), 0, h->stream, d_deps, h->d_depsT, nvar,
@@ -2481,16 +2504,13 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    }
    if (e == hipSuccess)
    {
-      if (h->nd <= 64 && h->pmaxc <= VMAXC)
-      {
-         hipLaunchKernelGGL(k_variant_rows_reg, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
-                            h->stream, V);
-      }
-      else
-      {
-         hipLaunchKernelGGL(k_variant_rows, dim3((h->height + 3) / 4), dim3(BLOCK), shmem,
-                            h->stream, V);
-      }
+      const dim3 grid((unsigned)((h->height + 3) / 4));
+      if (vk == 3) { hipLaunchKernelGGL(k_variant_rows, grid, dim3(BLOCK), shmem, h->stream, V); }
+      else if (vk == 2) { hipLaunchKernelGGL(k_variant_rows_reg, grid, dim3(BLOCK), shmem, h->stream, V); }
+      else if (ns <= 1) { hipLaunchKernelGGL(k_variant_rows_acc<1>, grid, dim3(BLOCK), shmem, h->stream, V); }
+      else if (ns == 2) { hipLaunchKernelGGL(k_variant_rows_acc<2>, grid, dim3(BLOCK), shmem, h->stream, V); }
+      else if (ns == 3) { hipLaunchKernelGGL(k_variant_rows_acc<3>, grid, dim3(BLOCK), shmem, h->stream, V); }
+      else { hipLaunchKernelGGL(k_variant_rows_acc<4>, grid, dim3(BLOCK), shmem, h->stream, V); }
       e = hipGetLastError();
    }
    if (e == hipSuccess) { e = ev_end(h, T_BATCH); }

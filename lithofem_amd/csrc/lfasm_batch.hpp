@@ -205,6 +205,130 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
    }
 }
 
+// Accumulating form of k_variant_rows_reg for rows of at most 64 NS entries
+// and nd <= 64.  The base row stays in registers (lane l holds entries l,
+// l + 64, ...); per variant the wave writes it into its row buffer, the
+// lanes of each contributor add their scaled unit-eps entries with LDS
+// floating-point atomics, and the row is stored.  One contributor's columns
+// are distinct slots and the contributors are issued in ascending element
+// order, so every slot sums base + its contributions in the same order as
+// above, without a wait between contributors.  The eps offsets are
+// wave-uniform loads through the scalar cache (parameter-major copy).  Lanes
+// without an entry (b >= nd, eliminated columns) add zero to a slot of their
+// own past the row (64 sink entries per wave).
+typedef const __attribute__((address_space(4))) double *const_dbl_ptr;
+
+template <int NS>
+__global__ void __launch_bounds__(BLOCK) k_variant_rows_acc(VariantParams V)
+{
+   extern __shared__ double2 vbuf[];
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + wave;
+   if (r >= V.n) { return; }
+   double2 *buf = vbuf + (size_t)wave * (V.maxrow + 64);
+   if (V.vcopied && V.vcopied[r]) { return; }   // k_variant_copy
+   const int base = V.Ip[r];
+   const int len = V.Ip[r + 1] - base;
+   const int c0 = V.prc_ptr[r], c1 = V.prc_ptr[r + 1];
+   double2 x[NS];
+#pragma unroll
+   for (int c = 0; c < NS; ++c)
+   {
+      const int s = lane + 64 * c;
+      x[c] = s < len ? V.base[base + s] : make_double2(0.0, 0.0);
+   }
+   if (c0 == c1 || (V.eliminate && V.is_ess[r]))
+   {
+      for (int v = 0; v < V.nvar; ++v)
+      {
+         double2 *o = V.out + (size_t)v * V.nnz + base;
+#pragma unroll
+         for (int c = 0; c < NS; ++c)
+         {
+            if (lane + 64 * c < len) { o[lane + 64 * c] = x[c]; }
+         }
+      }
+      return;
+   }
+   const int nd = V.nd;
+   const int nc = c1 - c0;                    // <= VMAXC (host-checked)
+   const unsigned char ess_bit = V.eliminate ? 2 : 0;
+   const int b = lane;
+   double2 m[VMAXC];
+   int sl[VMAXC], pidx[VMAXC];
+#pragma unroll
+   for (int k = 0; k < VMAXC; ++k)
+   {
+      m[k] = make_double2(0.0, 0.0);
+      sl[k] = V.maxrow + lane;
+      pidx[k] = 0;
+      if (k < nc)
+      {
+         const int pa = __builtin_amdgcn_readfirstlane(V.prc_list[c0 + k]);
+         const int pi = pa / nd, a = pa - pi * nd;
+         const int e = V.pelem[pi];
+         const int ea = e * nd + a;
+         pidx[k] = pi;
+         if (b < nd)
+         {
+            const unsigned char fa = V.eflag[ea];
+            const unsigned char fb = V.eflag[(size_t)e * nd + b];
+            if ((fb & ess_bit) == 0)
+            {
+               sl[k] = V.slot[(size_t)ea * V.sld + b];
+               const double2 y = V.Mhat[((size_t)pi * nd + a) * nd + b];
+               m[k] = ((fa ^ fb) & 1) ? make_double2(-y.x, -y.y) : y;
+            }
+         }
+      }
+   }
+   const const_dbl_ptr dT = (const_dbl_ptr)V.depsT;
+   for (int v = 0; v < V.nvar; ++v)
+   {
+      wave_sync_lds();
+#pragma unroll
+      for (int c = 0; c < NS; ++c)
+      {
+         if (lane + 64 * c < len) { buf[lane + 64 * c] = x[c]; }
+      }
+      // contributors in groups of 4: the group's offsets are loaded together
+      // (pidx = 0 past the row's contributors keeps the addresses valid)
+#pragma unroll
+      for (int k4 = 0; k4 < VMAXC; k4 += 4)
+      {
+         if (k4 < nc)
+         {
+            double2 d[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+               const size_t i = 2 * ((size_t)pidx[k4 + j] * V.nvar + v);
+               d[j] = make_double2(dT[i], dT[i + 1]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+               if (k4 + j < nc)
+               {
+                  const double2 t = cmul(d[j], m[k4 + j]);
+                  double *q = reinterpret_cast<double *>(buf + sl[k4 + j]);
+                  __hip_atomic_fetch_add(q, t.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                  __hip_atomic_fetch_add(q + 1, t.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+               }
+            }
+         }
+      }
+      wave_sync_lds();
+      double2 *o = V.out + (size_t)v * V.nnz + base;
+#pragma unroll
+      for (int c = 0; c < NS; ++c)
+      {
+         if (lane + 64 * c < len) { o[lane + 64 * c] = buf[lane + 64 * c]; }
+      }
+   }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
 {
    extern __shared__ double2 vbuf[];

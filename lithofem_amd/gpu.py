@@ -122,6 +122,7 @@ def lib():
         "lfasm_eps_sensitivity": ([vp, i, vp, vp, vp, i, vp], i),
         "lfasm_set_tensor_path": ([vp, i], i),
         "lfasm_set_element_concurrency": ([vp, i, i], i),
+        "lfasm_set_variant_kernel": ([vp, i], i),
         "lfasm_step_capture": ([vp, i, i, i, i], i),
         "lfasm_step_launch": ([vp, i], i),
         "lfasm_build_global_device": ([vp, vp], i),
@@ -444,6 +445,12 @@ class AsmGpuGlobal:
         the rows formed only by tensor-path elements are scattered beside
         the quadrature kernel (lfasm_set_element_concurrency)."""
         self._check(lib().lfasm_set_element_concurrency(self._h, int(grid), int(scatter_overlap)))
+
+    def set_variant_kernel(self, kernel: int) -> None:
+        """Form of the eps-variant row kernel: 0 automatic, 1 accumulating
+        (LDS atomics), 2 register (read-modify-write), 3 general
+        (lfasm_set_variant_kernel)."""
+        self._check(lib().lfasm_set_variant_kernel(self._h, int(kernel)))
 
     def stream(self) -> int:
         return lib().lfasm_stream(self._h) or 0

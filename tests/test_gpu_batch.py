@@ -120,6 +120,47 @@ def test_eps_variants_equal_plain_assembly(p):
         g.close()
 
 
+@pytest.mark.parametrize("eliminate", [True, False])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_variant_kernel_forms_agree(p, eliminate):
+    # the accumulating (1), register (2) and general (3) forms sum every
+    # entry as base + the contributions in ascending element order: the same
+    # bits; nd = 84 at p = 4 leaves only the general form
+    c = _hole(4, 44.0, 6.0, 30.0)
+    pb = c.problem
+    ad = build_asm_data(c.mesh, p)
+    I, J = csr_symbolic(ad.dof, ad.height)
+    params = np.nonzero(ad.attr == 2)[0].astype(np.int32)
+    rng = np.random.default_rng(11)
+    nvar = 70                      # two blocks of 64 in the register form
+    deps = rng.uniform(-0.3, 0.3, (nvar, len(params))) + 1j * rng.uniform(-0.1, 0.1, (nvar, len(params)))
+    g = gpu.AsmGpuGlobal(0)
+    try:
+        g.init(ad, pb)
+        g.build_global(I, J, ad.ess)
+        g.compute_local()
+        g.scatter_eliminate(eliminate)
+        g.variants_prepare(params)
+        out = {}
+        for k in (3, 2, 1, 0):
+            g.set_variant_kernel(k)
+            if p == 4 and k in (1, 2):
+                with pytest.raises(gpu.AsmGpuError) as ei:
+                    g.variants_assemble(deps, eliminate=eliminate)
+                assert ei.value.code == gpu.E_UNSUPPORTED
+                continue
+            g.variants_assemble(deps, eliminate=eliminate)
+            out[k] = [g.download_variant(v) for v in (0, 37, 64, nvar - 1)]
+        print(f"p={p} nparam={len(params)} forms={sorted(out)}", flush=True)
+        for k, vals in out.items():
+            for (a0, a1), (b0, b1) in zip(vals, out[3]):
+                assert np.array_equal(a0, b0) and np.array_equal(a1, b1), k
+        with pytest.raises(gpu.AsmGpuError):
+            g.set_variant_kernel(4)
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("p", [1, 2, 3])
 @pytest.mark.parametrize("name", ["grating", "hole"])
 def test_batched_rhs_matches_oracle(name, p):
