@@ -182,6 +182,9 @@ def main() -> int:
     ap.add_argument("--scatter-overlap", type=int, default=None, choices=[0, 1],
                     help="scatter the tensor-only rows beside the quadrature kernel in the "
                          "captured step (default: the autotuner's choice)")
+    ap.add_argument("--quad-residency", type=int, default=None,
+                    help="quadrature workgroups per CU beside the other kernels in the "
+                         "captured step (0 = no cap; default: the autotuner's choice)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-variant and adjoint figures of the default run")
     ap.add_argument("--mode", default="assemble",
@@ -272,16 +275,20 @@ def main() -> int:
     if args.concurrency is not None:
         conc = args.concurrency
     scat = 0
+    qres = 0
     tuned_step = None
     if args.scatter_overlap is not None:
         scat = args.scatter_overlap
     elif conc > 0 and args.graph == "on":
         from lithofem_amd.tune import tune_step
         tuned_step = tune_step(g, conc)
-        conc, scat = tuned_step["best"]
+        conc, scat, qres = tuned_step["best"]
         log(f"autotune: captured step (tensor cap, tensor-only rows scattered beside the "
-            f"quadrature kernel) = ({conc}, {scat}): " +
+            f"quadrature kernel, quadrature workgroups per CU) = ({conc}, {scat}, {qres}): " +
             ", ".join(f"{k}={v:.3f}ms" for k, v in tuned_step["ms"].items()))
+    if args.quad_residency is not None:
+        qres = args.quad_residency
+    g.set_quad_residency(qres)
     # the eager steps time every kernel on its own (element kernels in
     # sequence); the captured step runs with the chosen concurrency
     g.set_element_concurrency(0)
@@ -324,10 +331,12 @@ def main() -> int:
             ms_graph_seq = graph_ms()
             log(f"graph step, element kernels in sequence {ms_graph_seq:.3f} ms")
             g.set_element_concurrency(conc, scat)
+            g.set_quad_residency(qres)
         ms_step = graph_ms()
         log(f"graph step {ms_step:.3f} ms" +
             (f" (element kernels side by side, tensor kernel capped at {conc} workgroups; "
-             f"tensor-only rows scattered beside the quadrature kernel: {scat})"
+             f"tensor-only rows scattered beside the quadrature kernel: {scat}; quadrature "
+             f"workgroups per CU: {qres or 'no cap'})"
              if conc > 0 else ""))
     log(f"step {ms_step:.3f} ms (element matrices {tl_avg:.3f} ms = tensor {tt_avg:.3f} + "
         f"quadrature {tq_avg:.3f}; scatter {ts_avg:.3f} ms)")
@@ -411,7 +420,8 @@ def main() -> int:
                    else "eager",
                    "element_concurrency_grid": conc,
                    "step_scatter_overlap": scat,
-                   "step_overlap_ms": ({f"{k[0]},{k[1]}": v for k, v in tuned_step["ms"].items()}
+                   "step_quad_residency": qres,
+                   "step_overlap_ms": ({",".join(map(str, k)): v for k, v in tuned_step["ms"].items()}
                                        if tuned_step else None),
                    "element_concurrency_ms": ({str(k): v for k, v in
                                                tuned["concurrency"]["ms"].items()}

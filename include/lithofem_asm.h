@@ -223,6 +223,16 @@ int lfasm_set_tensor_path(lfasm_handle *h, int mode);
  * mesh.  Drops the captured step. */
 int lfasm_set_element_concurrency(lfasm_handle *h, int grid, int scatter_overlap);
 
+/* Residency cap of the MFMA quadrature kernels when they run beside the
+ * tensor kernel and the row scatter (no reference counterpart): with
+ * wg_per_cu > 0 (at most 8) and element kernels side by side
+ * (lfasm_set_element_concurrency grid > 0), a quadrature launch asks for
+ * enough LDS that at most wg_per_cu of its workgroups fit on one CU, so the
+ * HBM-bound kernels beside it keep more wave slots; 0 = no cap.  Launches in
+ * sequence are not affected, nor are values.  The launch autotuner chooses
+ * it with the concurrency.  Drops the captured step. */
+int lfasm_set_quad_residency(lfasm_handle *h, int wg_per_cu);
+
 /* Captured assembly step (no reference counterpart; the reference launches
  * ComputeLocal and ScatterEliminate one kernel at a time,
  * solver/lithofem_solve.cpp:789-790).  capture records the whole step --

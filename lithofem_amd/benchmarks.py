@@ -400,13 +400,14 @@ def large(log, n_xy=16, nz_scale=1.0, steps=5) -> dict:
             f"scatter {ts / steps:.2f})")
         # the captured step with the tuned overlap of the element kernels and
         # the scatters (or in sequence when the tuner keeps them apart)
-        overlap = (0, 0)
+        overlap = (0, 0, 0)
         if conc > 0:
             ts_ = tune_step(g, conc)
             overlap = ts_["best"]
-            log(f"large: captured step (tensor cap, scatter overlap) = {overlap}: " +
-                ", ".join(f"{k}={v:.2f}ms" for k, v in ts_["ms"].items()))
-        g.set_element_concurrency(*overlap)
+            log(f"large: captured step (tensor cap, scatter overlap, quadrature residency) = "
+                f"{overlap}: " + ", ".join(f"{k}={v:.2f}ms" for k, v in ts_["ms"].items()))
+        g.set_element_concurrency(overlap[0], overlap[1])
+        g.set_quad_residency(overlap[2])
         g.step_capture()
         g.step_launch(sync=True)
         t0 = time.perf_counter()

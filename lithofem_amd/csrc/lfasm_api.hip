@@ -263,6 +263,7 @@ struct lfasm_handle {
    // conc_grid workgroups along the element chunks, the quadrature kernel on
    // stream2 (both default priority), forked and joined by events
    int conc_grid = 0;
+   int quad_res = 0;       // quadrature workgroups per CU beside the other kernels (0 = no cap)
    // captured step only (scat_overlap = 1): rows whose contributors are all
    // tensor-path elements (rows_t) are scattered on the tensor branch while
    // the quadrature kernel runs; the other rows (rows_q) after the join
@@ -456,6 +457,21 @@ hipError_t launch_elem_prep(lfasm_handle *h, const LocalParams &P, hipStream_t s
    return hipGetLastError();
 }
 
+constexpr size_t LDS_MAX = 163840;   // LDS bytes per CU on gfx950
+
+// LDS request of an MFMA quadrature kernel.  Launched on the second stream
+// (beside the tensor kernel and the row scatter) with a residency cap
+// quad_res > 0, the request is raised so that at most quad_res workgroups fit
+// on one CU: the kernel then holds fewer of the CU's wave slots and vector
+// registers, which the HBM-bound kernels beside it need to keep enough
+// loads and stores in flight.  The kernel uses only its own part.
+size_t resident_shmem(const lfasm_handle *h, size_t shmem, hipStream_t st)
+{
+   if (st != h->stream2 || h->quad_res <= 0) { return shmem; }
+   const size_t need = LDS_MAX / (size_t)(h->quad_res + 1) + 1024;
+   return std::max(shmem, std::min(need, h->lds_max));
+}
+
 template <int NT, int RT, int EPB, int WPEU>
 hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P, hipStream_t st)
 {
@@ -463,7 +479,8 @@ hipError_t launch_mfma(lfasm_handle *h, const LocalParams &P, hipStream_t st)
    constexpr int WPE = NT / RT;
    constexpr int THREADS = 64 * WPE * EPB;
    const int nqt = P.rc.nq + P.rm.nq;
-   const size_t shmem = (size_t)EPB * (EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP) * sizeof(double);
+   const size_t shmem = resident_shmem(
+      h, (size_t)EPB * (EC_SIZE + nqt * 4 + 3 * CHUNK_ROWS * NP) * sizeof(double), st);
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma<NT, RT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
@@ -498,8 +515,8 @@ hipError_t launch_mfma_pipe(lfasm_handle *h, const LocalParams &P, hipStream_t s
    constexpr int NP = 16 * NT;
    constexpr int THREADS = 64 * NT * EPB;
    const int nqt = P.rc.nq + P.rm.nq;
-   const size_t shmem =
-      (size_t)EPB * (EC_SIZE + nqt * 4 + 2 * 3 * CHUNK_ROWS * NP) * sizeof(double);
+   const size_t shmem = resident_shmem(
+      h, (size_t)EPB * (EC_SIZE + nqt * 4 + 2 * 3 * CHUNK_ROWS * NP) * sizeof(double), st);
    hipError_t e = hipFuncSetAttribute((const void *)k_local_mfma_pipe<NT, EPB, WPEU>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e != hipSuccess) { return e; }
@@ -524,7 +541,8 @@ hipError_t launch_mfma_sym(lfasm_handle *h, const LocalParams &P, hipStream_t st
    {
       return launch_mfma_sym<NBUF, WPEU, PRIO, false>(h, P, st);
    }
-   const size_t shmem = (size_t)(EC_SIZE + nqt * 4 + SYM_FCS + NBUF * SYM_STG) * sizeof(double);
+   const size_t shmem = resident_shmem(
+      h, (size_t)(EC_SIZE + nqt * 4 + SYM_FCS + NBUF * SYM_STG) * sizeof(double), st);
    auto kern = k_local_mfma_sym<NBUF, WPEU, PRIO, PERSIST>;
    hipError_t e = hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
@@ -625,8 +643,6 @@ void slice_geometry(int nd, int p, int t, int &r0, int &nR)
    r0 = t == 0 ? 0 : t == 1 ? ne_ : ne_ + nf;
    nR = t == 0 ? ne_ : t == 1 ? nf : nd - ne_ - nf;
 }
-
-constexpr size_t LDS_MAX = 163840;   // LDS bytes per CU on gfx950
 
 size_t slice_bytes(int nd, int nR)
 {
@@ -2014,6 +2030,18 @@ int lfasm_set_element_concurrency(lfasm_handle *h, int grid, int scatter_overlap
    }
    h->conc_grid = grid;
    h->scat_overlap = scatter_overlap;
+   drop_graph(h);
+   return LFASM_OK;
+}
+
+int lfasm_set_quad_residency(lfasm_handle *h, int wg_per_cu)
+{
+   if (!h) { return LFASM_E_STATE; }
+   if (wg_per_cu < 0 || wg_per_cu > 8)
+   {
+      return h->fail(LFASM_E_ARG, "quadrature residency %d out of range", wg_per_cu);
+   }
+   h->quad_res = wg_per_cu;
    drop_graph(h);
    return LFASM_OK;
 }

@@ -123,6 +123,7 @@ def lib():
         "lfasm_set_tensor_path": ([vp, i], i),
         "lfasm_set_element_concurrency": ([vp, i, i], i),
         "lfasm_set_variant_kernel": ([vp, i], i),
+        "lfasm_set_quad_residency": ([vp, i], i),
         "lfasm_step_capture": ([vp, i, i, i, i], i),
         "lfasm_step_launch": ([vp, i], i),
         "lfasm_build_global_device": ([vp, vp], i),
@@ -445,6 +446,11 @@ class AsmGpuGlobal:
         the rows formed only by tensor-path elements are scattered beside
         the quadrature kernel (lfasm_set_element_concurrency)."""
         self._check(lib().lfasm_set_element_concurrency(self._h, int(grid), int(scatter_overlap)))
+
+    def set_quad_residency(self, wg_per_cu: int) -> None:
+        """At most `wg_per_cu` quadrature workgroups per CU when the element
+        kernels run side by side (0 = no cap; lfasm_set_quad_residency)."""
+        self._check(lib().lfasm_set_quad_residency(self._h, int(wg_per_cu)))
 
     def set_variant_kernel(self, kernel: int) -> None:
         """Form of the eps-variant row kernel: 0 automatic, 1 accumulating

@@ -5,7 +5,8 @@ VALU kernel and the MFMA variants: elements per workgroup, waves per EU,
 pipelined or not) on the handle's own mesh with HIP events, and makes the
 fastest one the handle's default; then, when the reference-tensor kernel
 and the quadrature kernel both run, the workgroup cap under which they run
-side by side (0 = in sequence).  Results are plain dicts so callers can log
+side by side (0 = in sequence), and for the captured step the overlap of
+the row scatter and the quadrature kernel's residency per CU.  Results are plain dicts so callers can log
 them.
 """
 
@@ -71,6 +72,9 @@ def tune_concurrency(g: "gpu.AsmGpuGlobal", reps: int = 3, mask_cc: int = gpu.CU
 
 
 
+QUAD_RESIDENCIES = (0, 3, 2)   # quadrature workgroups per CU beside the other kernels
+
+
 def tune_step(g: "gpu.AsmGpuGlobal", grid: int, reps: int = 5,
               mask_cc: int = gpu.CURLCURL, mask_mass: int = gpu.MASS_ALL) -> dict:
     """Times the captured step (element matrices, scatter, elimination) with
@@ -78,20 +82,25 @@ def tune_step(g: "gpu.AsmGpuGlobal", grid: int, reps: int = 5,
     rows formed by tensor-path elements only scattered beside the
     quadrature kernel -- the latter also with wider tensor caps, under which
     the quadrature kernel runs more beside that scatter than beside the
-    tensor kernel -- and keeps the fastest (grid, overlap) pair."""
+    tensor kernel -- each with the quadrature kernel's residency uncapped or
+    held to 3 or 2 workgroups per CU, and keeps the fastest
+    (grid, overlap, residency) triple."""
     ms = {}
     cands = [(grid, 0), (grid, 1)]
     if grid > 0:
-        cands += [(2 * grid, 1), (4 * grid, 1), (1 << 20, 1)]
-    for gr, sg in cands:
-        g.set_element_concurrency(gr, sg)
-        g.step_capture(mask_cc, mask_mass, True, True)
-        g.step_launch(sync=True)                    # warm-up replay
-        best = float("inf")
-        for _ in range(reps):
-            g.step_launch(sync=True)
-            best = min(best, g.timings()["step_ms"])
-        ms[(gr, sg)] = best
+        cands += [(2 * grid, 1), (4 * grid, 1), (8 * grid, 1), (1 << 20, 1)]
+    for res in QUAD_RESIDENCIES:
+        g.set_quad_residency(res)
+        for gr, sg in cands:
+            g.set_element_concurrency(gr, sg)
+            g.step_capture(mask_cc, mask_mass, True, True)
+            g.step_launch(sync=True)                    # warm-up replay
+            best = float("inf")
+            for _ in range(reps):
+                g.step_launch(sync=True)
+                best = min(best, g.timings()["step_ms"])
+            ms[(gr, sg, res)] = best
     choice = min(ms, key=ms.get)
-    g.set_element_concurrency(*choice)
+    g.set_element_concurrency(choice[0], choice[1])
+    g.set_quad_residency(choice[2])
     return {"best": choice, "ms": ms}

@@ -3,7 +3,8 @@ reference-tensor kernel on the handle's stream, held to a workgroup cap,
 and the quadrature kernel on a second stream give exactly the values of
 the two kernels in sequence, eager and as captured-step replays; so does
 the captured step that scatters the rows formed only by tensor-path
-elements beside the quadrature kernel."""
+elements beside the quadrature kernel, and the quadrature kernel held to a
+few workgroups per CU (lfasm_set_quad_residency)."""
 
 from __future__ import annotations
 
@@ -52,8 +53,39 @@ def test_concurrent_element_kernels_match_sequence(conc):
         for scat in (1,):
             got = _assemble(g, conc, True, scat)
             assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        # the quadrature kernel held to 3, 2 and 1 workgroups per CU beside
+        # the other kernels, eager and in both captured forms
+        for res in (3, 2, 1):
+            g.set_quad_residency(res)
+            for graph, scat in ((False, 0), (True, 0), (True, 1)):
+                got = _assemble(g, conc, graph, scat)
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        g.set_quad_residency(0)
+        with pytest.raises(gpu.AsmGpuError):
+            g.set_quad_residency(9)
         # back to the sequence, captured
         got = _assemble(g, 0, True)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_quad_residency_general_kernels(p):
+    # the general MFMA kernels (orders 2 and 4) under a residency cap beside
+    # the tensor kernel give the values of the sequence
+    c, ad, I, J = case("hole", p, 6.0, "small")
+    print(f"residency, order {p}: ne={ad.ne} nnz={len(J)}", flush=True)
+    g = gpu.AsmGpuGlobal(0)
+    try:
+        g.init(ad, c.problem)
+        g.build_global(I, J, ad.ess)
+        assert g.set_tensor_path(1) > 0
+        ref = _assemble(g, 0, False)
+        for res in (0, 2):
+            g.set_quad_residency(res)
+            for graph, scat in ((False, 0), (True, 1)):
+                got = _assemble(g, 16, graph, scat)
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
     finally:
         g.close()
