@@ -318,6 +318,28 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
  * contributions in ascending element order.  assemble fails with
  * LFASM_E_UNSUPPORTED when a forced form does not fit the mesh. */
 int lfasm_set_variant_kernel(lfasm_handle *h, int kernel);
+/* ---- angle sweeps (no reference counterpart; the reference assembles one
+ * matrix per incidence angle, solver/lithofem_solve.cpp:744-808) ----------
+ * For one mesh and wavelength the k_par terms are polynomial in
+ * k_par = (kx, ky): CROSS0 + CROSS1 (i K Lambda^-1 + i Lambda^-1 K) are linear
+ * in K and KLK (-K Lambda^-1 K) quadratic, with K linear in (kx, ky).  So
+ *   A(kx, ky) = B0 + kx Bx + ky By + kx^2 Bxx + ky^2 Byy + kx ky (B11 - Bxx - Byy)
+ * where B0 holds the curl-curl and eps-mass terms of the masks, Bx / By the
+ * cross terms at k_par = (1, 0) / (0, 1) and Bxx / Byy / B11 the KLK term at
+ * (1, 0) / (0, 1) / (1, 1) (By, Byy, B11 only when some ky != 0).
+ * angles_assemble forms those basis matrices (element matrices + scatter +
+ * `eliminate`, on a single-mesh handle after build_global), then every
+ * angle's CSR values in one pass over the nonzeros; kxy = nang x (kx, ky).
+ * Eliminated rows are (0 ... 1 ... 0) in every angle.  The values equal a
+ * plain assembly at that k_par up to rounding (the same terms, summed in a
+ * different order).  The handle's own k_par is restored; its element
+ * matrices are left holding a basis term, so call compute_local before the
+ * next scatter_eliminate. */
+int lfasm_angles_assemble(lfasm_handle *h, int nang, const double *kxy, int mask_cc, int mask_mass,
+                          int apply_te, int eliminate);
+int lfasm_angle_values(lfasm_handle *h, int j, void **dev_vals);
+int lfasm_download_angle(lfasm_handle *h, int j, double *re, double *im);
+
 int lfasm_variant_values(lfasm_handle *h, int v, void **dev_vals);
 int lfasm_download_variant(lfasm_handle *h, int v, double *re, double *im);
 

@@ -230,6 +230,13 @@ struct lfasm_handle {
    int nparam = 0, nvar = 0;
    int *pelem = nullptr, *prc_ptr = nullptr, *prc_list = nullptr;
    double2 *Mhat = nullptr, *var_vals = nullptr;
+   // angle sweeps (lfasm_angles_assemble)
+   int nang = 0, nkb = 0;
+   double2 *ang_vals = nullptr;       // nang x nnz
+   double2 *kbasis = nullptr;         // nkb x nnz basis matrices
+   double *d_kcoef = nullptr;         // nang x 6 combination coefficients
+   int kcoef_cap = 0;
+   std::vector<PmlDev> pml_host;      // host copy of pml_tab (k_par of each mesh)
    std::vector<int> pelem_host;
    std::vector<int> dof_host;         // element dof table (host copy)
    int pmaxc = 0;                     // most parameter contributors of one row
@@ -377,6 +384,8 @@ void release_global(lfasm_handle *h)
    h->nrows_t = h->nrows_q = 0;
    h->rows_split = false;
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
+   dfree(h->ang_vals); dfree(h->kbasis); dfree(h->d_kcoef);
+   h->nang = h->nkb = h->kcoef_cap = 0;
    dfree(h->rhs); dfree(h->d_deps); dfree(h->d_depsT); dfree(h->vfix); dfree(h->vcopied);
    h->nvfix = 0;
    h->rhs_cap = 0;
@@ -1518,6 +1527,7 @@ int init_impl(lfasm_handle *h, const lfasm_mesh *ms, int nmesh, const double *xy
                           (const double2 *)eps_tab.data() + eps_tab.size() / 2);
    LF_HIP(upload(h, h->eps_off, eps_off.data(), eps_off.size()));
    LF_HIP(upload(h, h->pml_tab, pml.data(), pml.size()));
+   h->pml_host = pml;
    std::memcpy(h->TInv, m0.TInv, sizeof(h->TInv));
    LF_HIP(upload(h, h->TInv_dev, h->TInv, 24));
    // element-matrix rows padded to whole 128-byte lines (8 entries)
@@ -2545,6 +2555,144 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
    LF_HIP(e);
    return LFASM_OK;
+}
+
+// k_par of the handle's (single) mesh, on the device table and the host copy
+hipError_t set_kpar(lfasm_handle *h, double kx, double ky)
+{
+   h->pml_host[0].kx = kx;
+   h->pml_host[0].ky = ky;
+   h->any_kpar = kx != 0.0 || ky != 0.0;
+   LF_TRY(hipMemcpyAsync(h->pml_tab, h->pml_host.data(), sizeof(PmlDev), hipMemcpyHostToDevice,
+                         h->stream));
+   return hipStreamSynchronize(h->stream);
+}
+
+int lfasm_angles_assemble(lfasm_handle *h, int nang, const double *kxy, int mask_cc, int mask_mass,
+                          int apply_te, int eliminate)
+{
+   if (!h || !h->inited || !h->global_built)
+   {
+      return h ? h->fail(LFASM_E_STATE, "init and build_global first") : LFASM_E_STATE;
+   }
+   if (h->nmesh != 1) { return h->fail(LFASM_E_UNSUPPORTED, "angle sweeps take a single-mesh handle"); }
+   if (nang <= 0 || !kxy) { return h->fail(LFASM_E_ARG, "no angles"); }
+   LF_HIP(hipSetDevice(h->device));
+   drop_graph(h);
+   bool any_ky = false;
+   for (int j = 0; j < nang; ++j) { any_ky = any_ky || kxy[2 * j + 1] != 0.0; }
+   // basis matrices: (mask_cc, mask_mass, kx, ky) of each
+   struct Basis { int mcc, mm; double kx, ky; };
+   const int cross = mask_mass & (LFASM_CROSS0 | LFASM_CROSS1), klk = mask_mass & LFASM_KLK;
+   std::vector<Basis> bs;
+   bs.push_back({mask_cc, mask_mass & LFASM_MASS, 0.0, 0.0});
+   const int ix = cross ? (int)bs.size() : -1;
+   if (cross) { bs.push_back({0, cross, 1.0, 0.0}); }
+   const int ixx = klk ? (int)bs.size() : -1;
+   if (klk) { bs.push_back({0, klk, 1.0, 0.0}); }
+   const int iy = (cross && any_ky) ? (int)bs.size() : -1;
+   if (iy >= 0) { bs.push_back({0, cross, 0.0, 1.0}); }
+   const int iyy = (klk && any_ky) ? (int)bs.size() : -1;
+   if (iyy >= 0) { bs.push_back({0, klk, 0.0, 1.0}); }
+   const int i11 = (klk && any_ky) ? (int)bs.size() : -1;
+   if (i11 >= 0) { bs.push_back({0, klk, 1.0, 1.0}); }
+   const int nb = (int)bs.size();
+   // combination coefficients per angle (basis order)
+   std::vector<double> c((size_t)nang * nb, 0.0);
+   for (int j = 0; j < nang; ++j)
+   {
+      const double kx = kxy[2 * j], ky = kxy[2 * j + 1];
+      double *cj = c.data() + (size_t)j * nb;
+      cj[0] = 1.0;
+      if (ix >= 0) { cj[ix] = kx; }
+      if (iy >= 0) { cj[iy] = ky; }
+      if (ixx >= 0) { cj[ixx] = i11 >= 0 ? kx * kx - kx * ky : kx * kx; }
+      if (iyy >= 0) { cj[iyy] = ky * ky - kx * ky; }
+      if (i11 >= 0) { cj[i11] = kx * ky; }
+   }
+   const size_t nnz = (size_t)h->nnz;
+   if (nang != h->nang)
+   {
+      dfree(h->ang_vals);
+      h->nang = 0;
+      LF_HIP(hipMalloc((void **)&h->ang_vals, (size_t)nang * nnz * sizeof(double2)));
+      h->nang = nang;
+   }
+   if (nb > h->nkb)
+   {
+      dfree(h->kbasis);
+      h->nkb = 0;
+      LF_HIP(hipMalloc((void **)&h->kbasis, (size_t)nb * nnz * sizeof(double2)));
+      h->nkb = nb;
+   }
+   if ((int)c.size() > h->kcoef_cap)
+   {
+      dfree(h->d_kcoef);
+      h->kcoef_cap = 0;
+      LF_HIP(hipMalloc((void **)&h->d_kcoef, c.size() * sizeof(double)));
+      h->kcoef_cap = (int)c.size();
+   }
+   LF_HIP(hipMemcpyAsync(h->d_kcoef, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice,
+                         h->stream));
+   const double kx0 = h->pml_host[0].kx, ky0 = h->pml_host[0].ky;
+   hipError_t e = ev_begin(h, T_BATCH);
+   // each basis matrix: element matrices, then the scatter into its own
+   // buffer (the handle's value array swapped for it)
+   double2 *vals = h->vals;
+   for (int b = 0; b < nb && e == hipSuccess; ++b)
+   {
+      e = set_kpar(h, bs[b].kx, bs[b].ky);
+      if (e != hipSuccess) { break; }
+      const LocalParams P = local_params(h, bs[b].mcc, bs[b].mm, apply_te, h->Ae);
+      e = enqueue_local(h, P, 0, false);
+      if (e != hipSuccess) { break; }
+      h->vals = h->kbasis + (size_t)b * nnz;
+      e = enqueue_scatter(h, eliminate, false);
+      h->vals = vals;
+   }
+   if (e == hipSuccess)
+   {
+      KparCombine K;
+      K.nnz = (int64_t)nnz;
+      K.nb = nb;
+      K.nang = nang;
+      for (int q = 0; q < 6; ++q) { K.B[q] = h->kbasis + (size_t)std::min(q, nb - 1) * nnz; }
+      K.coef = h->d_kcoef;
+      K.out = h->ang_vals;
+      const unsigned grid = (unsigned)std::min<size_t>((nnz + 255) / 256, (size_t)h->cus * 16);
+      hipLaunchKernelGGL(k_kpar_combine, dim3(std::max(1u, grid)), dim3(256), 0, h->stream, K);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess && eliminate)
+   {
+      hipLaunchKernelGGL(k_kpar_ess, dim3((unsigned)std::min(8192, (h->height + 255) / 256)), dim3(256),
+                         0, h->stream, h->height, (const int *)h->Ip, (const unsigned char *)h->ess,
+                         (const double2 *)h->kbasis, h->ang_vals, (int64_t)nnz, nang);
+      e = hipGetLastError();
+   }
+   if (e == hipSuccess) { e = ev_end(h, T_BATCH); }
+   if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }
+   // the handle's own k_par back; its element matrices now hold the last
+   // basis term, so the next scatter needs a compute_local first
+   const hipError_t e2 = set_kpar(h, kx0, ky0);
+   h->local_valid = false;
+   LF_HIP(e);
+   LF_HIP(e2);
+   return LFASM_OK;
+}
+
+int lfasm_angle_values(lfasm_handle *h, int j, void **dev_vals)
+{
+   if (!h || j < 0 || j >= h->nang || !dev_vals) { return h ? h->fail(LFASM_E_ARG, "angle index") : LFASM_E_STATE; }
+   *dev_vals = (void *)(h->ang_vals + (size_t)j * h->nnz);
+   return LFASM_OK;
+}
+
+int lfasm_download_angle(lfasm_handle *h, int j, double *re, double *im)
+{
+   if (!h || j < 0 || j >= h->nang) { return h ? h->fail(LFASM_E_ARG, "angle index") : LFASM_E_STATE; }
+   LF_HIP(hipSetDevice(h->device));
+   return download_split(h, h->ang_vals + (size_t)j * h->nnz, (size_t)h->nnz, re, im);
 }
 
 int lfasm_variant_values(lfasm_handle *h, int v, void **dev_vals)

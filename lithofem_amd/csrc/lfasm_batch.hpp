@@ -392,6 +392,69 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows(VariantParams V)
    }
 }
 
+// ---- angle sweeps through the k_par structure --------------------------------
+//
+// K(kx, ky) is linear in (kx, ky), CROSS0 + CROSS1 linear in K and KLK
+// quadratic, so for one mesh and wavelength the system matrix of every
+// incidence angle is
+//   A(kx, ky) = B0 + kx Bx + ky By + kx^2 Bxx + ky^2 Byy + kx ky (B11 - Bxx - Byy)
+// with B0 the curl-curl + eps-mass matrix, Bx / By the cross terms at
+// k_par = (1, 0) / (0, 1), Bxx / Byy / B11 the KLK term at (1, 0) / (0, 1) /
+// (1, 1); the host passes per angle one coefficient per basis matrix.  One
+// pass over the nonzeros reads the nb basis values once and writes every
+// angle's value (sums in basis order).
+
+struct KparCombine {
+   int64_t nnz;
+   int nb, nang;
+   const double2 *B[6];
+   const double *coef;   // nang x nb
+   double2 *out;         // nang x nnz
+};
+
+__global__ void __launch_bounds__(256) k_kpar_combine(KparCombine K)
+{
+   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < K.nnz;
+        i += (int64_t)gridDim.x * 256)
+   {
+      double2 b[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) { b[q] = q < K.nb ? K.B[q][i] : make_double2(0.0, 0.0); }
+      for (int j = 0; j < K.nang; ++j)
+      {
+         const double *c = K.coef + (size_t)j * K.nb;
+         double re = b[0].x * c[0], im = b[0].y * c[0];
+#pragma unroll
+         for (int q = 1; q < 6; ++q)
+         {
+            if (q < K.nb)
+            {
+               re = __builtin_fma(c[q], b[q].x, re);
+               im = __builtin_fma(c[q], b[q].y, im);
+            }
+         }
+         __builtin_nontemporal_store(dbl2v{re, im},
+                                     reinterpret_cast<dbl2v *>(K.out + (size_t)j * K.nnz + i));
+      }
+   }
+}
+
+// Eliminated rows: every basis matrix holds the identity row there, so the
+// combination is replaced by B0's row (0 ... 1 ... 0) in every angle.
+__global__ void k_kpar_ess(int n, const int *__restrict__ Ip, const unsigned char *__restrict__ ess,
+                           const double2 *__restrict__ B0, double2 *__restrict__ out, int64_t nnz,
+                           int nang)
+{
+   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+   {
+      if (!ess[r]) { continue; }
+      for (int s = Ip[r]; s < Ip[r + 1]; ++s)
+      {
+         for (int j = 0; j < nang; ++j) { out[(size_t)j * nnz + s] = B0[s]; }
+      }
+   }
+}
+
 // ---- incident field ---------------------------------------------------------------
 
 // One slab wave (the reference's SlabWave, solver/problem_common.inc:11-16):

@@ -124,6 +124,9 @@ def lib():
         "lfasm_set_element_concurrency": ([vp, i, i], i),
         "lfasm_set_variant_kernel": ([vp, i], i),
         "lfasm_set_quad_residency": ([vp, i], i),
+        "lfasm_angles_assemble": ([vp, i, vp, i, i, i, i], i),
+        "lfasm_angle_values": ([vp, i, ct.POINTER(vp)], i),
+        "lfasm_download_angle": ([vp, i, vp, vp], i),
         "lfasm_step_capture": ([vp, i, i, i, i], i),
         "lfasm_step_launch": ([vp, i], i),
         "lfasm_build_global_device": ([vp, vp], i),
@@ -374,6 +377,23 @@ class AsmGpuGlobal:
         re = np.empty(self.nnz)
         im = np.empty(self.nnz)
         self._check(lib().lfasm_download_variant(self._h, v, _p(re), _p(im)))
+        return re, im
+
+    # -- angle sweeps
+    def angles_assemble(self, kxy, mask_cc=CURLCURL, mask_mass=MASS_ALL, apply_te=True,
+                        eliminate=True):
+        """CSR values of every k_par = (kx, ky) in `kxy` (nang x 2) from the
+        basis matrices of the k_par polynomial (lfasm_angles_assemble)."""
+        k = np.ascontiguousarray(np.asarray(kxy, dtype=np.float64).reshape(-1, 2))
+        self._check(lib().lfasm_angles_assemble(self._h, k.shape[0], _p(k), int(mask_cc),
+                                                int(mask_mass), 1 if apply_te else 0,
+                                                1 if eliminate else 0))
+        self.nang = k.shape[0]
+
+    def download_angle(self, j):
+        re = np.empty(self.nnz)
+        im = np.empty(self.nnz)
+        self._check(lib().lfasm_download_angle(self._h, j, _p(re), _p(im)))
         return re, im
 
     # -- incident fields, RHS, sensitivity

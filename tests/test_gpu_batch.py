@@ -3,11 +3,14 @@
 * block-diagonal multi-mesh batches (meshes of different size, PML, k_par)
   equal the meshes assembled one by one, bit for bit;
 * eps-variant delta assembly equals plain assembly with per-element eps;
+* angle sweeps from the k_par basis matrices equal plain assembly per angle;
 * batched incident-field RHS equals the oracle's ScatterSource assembly;
 * the eps-sensitivity kernel equals the oracle's lam^T (dA u - db).
 """
 
 from __future__ import annotations
+
+import copy
 
 import numpy as np
 import pytest
@@ -16,7 +19,7 @@ from cases import assert_parity, maxnorm_err
 from lithofem_amd import gpu
 from lithofem_amd.fespace import build_asm_data, csr_symbolic
 from lithofem_amd.incident import incident_set, sp_coefficients
-from lithofem_amd.problem import contact_hole, line_grating
+from lithofem_amd.problem import contact_hole, k_parallel, line_grating
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -118,6 +121,77 @@ def test_eps_variants_equal_plain_assembly(p):
             assert err < 1e-13, err
     finally:
         g.close()
+
+
+def _at_kpar(pb, kx, ky):
+    q = copy.deepcopy(pb)
+    q.kx, q.ky = kx, ky
+    return q
+
+
+@pytest.mark.parametrize("p", [2, 3])
+@pytest.mark.parametrize("name", ["grating", "hole"])
+def test_angle_sweep_matches_plain_assembly(name, p):
+    # k_par polynomial: every angle's matrix from 3 (phi = 0) or 6 basis
+    # matrices equals its plain assembly to rounding, and the oracle at
+    # the north star's tolerance; theta = 0 (no k_par terms) included
+    c = line_grating(nx=4, ny=3, nz_scale=0.15, theta=6.0) if name == "grating" else \
+        _hole(4, 44.0, 6.0, 30.0)
+    pb = c.problem
+    ad = build_asm_data(c.mesh, p)
+    I, J = csr_symbolic(ad.dof, ad.height)
+    phi = 0.0 if name == "grating" else 30.0
+    kxy = np.array([k_parallel(th, phi)[1:] for th in (0.0, 3.0, 9.0, 14.0)])
+    print(f"angle sweep {name} p={p}: ne={ad.ne} nnz={len(J)} kxy={kxy.tolist()}", flush=True)
+    g = gpu.AsmGpuGlobal(0)
+    try:
+        g.init(ad, pb)
+        g.build_global(I, J, ad.ess)
+        for elim in (True, False):
+            g.angles_assemble(kxy, eliminate=elim)
+            got = [g.download_angle(j) for j in range(len(kxy))]
+            if elim:
+                got_elim = got
+            for j, (kx, ky) in enumerate(kxy):
+                s1 = gpu.AsmGpuGlobal(0)
+                try:
+                    s1.init(ad, _at_kpar(pb, kx, ky))
+                    s1.build_global(I, J, ad.ess)
+                    s1.compute_local()
+                    s1.scatter_eliminate(elim)
+                    want = s1.download_values()
+                finally:
+                    s1.close()
+                err = maxnorm_err(got[j][0] + 1j * got[j][1], want[0] + 1j * want[1])
+                print(f"  eliminate {elim} angle {j}: vs plain {err:.2e}", flush=True)
+                assert err < 1e-14, (elim, j, err)
+        # the handle's own k_par is restored: a plain assembly matches its
+        # own fresh handle bit for bit
+        g.compute_local()
+        g.scatter_eliminate(True)
+        own = g.download_values()
+        f = gpu.AsmGpuGlobal(0)
+        try:
+            f.init(ad, pb)
+            f.build_global(I, J, ad.ess)
+            f.compute_local()
+            f.scatter_eliminate(True)
+            fresh = f.download_values()
+        finally:
+            f.close()
+        assert np.array_equal(own[0], fresh[0]) and np.array_equal(own[1], fresh[1])
+    finally:
+        g.close()
+    # one angle against the oracle (eliminated)
+    j = 2
+    q = _at_kpar(pb, *kxy[j])
+    orc = O.Oracle(ad, q, I, J)
+    rre, rim = orc.assemble()
+    is_ess = np.zeros(ad.height, np.uint8)
+    is_ess[ad.ess] = 1
+    rre, rim = O.eliminate(I, J, is_ess, rre, rim)
+    sc = O.eliminate_scale(I, J, is_ess, orc.scale_csr(4))
+    assert_parity(got_elim[j][0] + 1j * got_elim[j][1], rre + 1j * rim, 1e-13, f"angle {j}", sc)
 
 
 @pytest.mark.parametrize("eliminate", [True, False])
