@@ -234,6 +234,8 @@ struct lfasm_handle {
    int nang = 0, nkb = 0;
    double2 *ang_vals = nullptr;       // nang x nnz
    double2 *kbasis = nullptr;         // nkb x nnz basis matrices
+   double2 *kAe = nullptr;            // nkae element-matrix sets of the basis terms 1..
+   int nkae = 0;
    double *d_kcoef = nullptr;         // nang x 6 combination coefficients
    int kcoef_cap = 0;
    std::vector<PmlDev> pml_host;      // host copy of pml_tab (k_par of each mesh)
@@ -384,8 +386,8 @@ void release_global(lfasm_handle *h)
    h->nrows_t = h->nrows_q = 0;
    h->rows_split = false;
    dfree(h->pelem); dfree(h->prc_ptr); dfree(h->prc_list); dfree(h->Mhat); dfree(h->var_vals);
-   dfree(h->ang_vals); dfree(h->kbasis); dfree(h->d_kcoef);
-   h->nang = h->nkb = h->kcoef_cap = 0;
+   dfree(h->ang_vals); dfree(h->kbasis); dfree(h->d_kcoef); dfree(h->kAe);
+   h->nang = h->nkb = h->kcoef_cap = h->nkae = 0;
    dfree(h->rhs); dfree(h->d_deps); dfree(h->d_depsT); dfree(h->vfix); dfree(h->vcopied);
    h->nvfix = 0;
    h->rhs_cap = 0;
@@ -2618,7 +2620,21 @@ int lfasm_angles_assemble(lfasm_handle *h, int nang, const double *kxy, int mask
       LF_HIP(hipMalloc((void **)&h->ang_vals, (size_t)nang * nnz * sizeof(double2)));
       h->nang = nang;
    }
-   if (nb > h->nkb)
+   // one pass for all basis terms (their element matrices in separate
+   // sets, scatter and combination fused) when the row buffers of all terms
+   // fit in LDS; else each term's CSR values, then the combination
+   const size_t ae_elems = (size_t)h->ne * h->nd * h->lda;
+   const size_t shmem_multi = (size_t)4 * nb * h->maxrow * sizeof(double2);
+   const bool multi = h->rslot && h->emask && h->nd <= 64 && h->tensor_mode != 2 &&
+                      shmem_multi <= h->lds_max;
+   if (multi && nb - 1 > h->nkae)
+   {
+      dfree(h->kAe);
+      h->nkae = 0;
+      LF_HIP(hipMalloc((void **)&h->kAe, (size_t)(nb - 1) * ae_elems * sizeof(double2)));
+      h->nkae = nb - 1;
+   }
+   if (!multi && nb > h->nkb)
    {
       dfree(h->kbasis);
       h->nkb = 0;
@@ -2636,21 +2652,70 @@ int lfasm_angles_assemble(lfasm_handle *h, int nang, const double *kxy, int mask
                          h->stream));
    const double kx0 = h->pml_host[0].kx, ky0 = h->pml_host[0].ky;
    hipError_t e = ev_begin(h, T_BATCH);
-   // each basis matrix: element matrices, then the scatter into its own
-   // buffer (the handle's value array swapped for it)
    double2 *vals = h->vals;
    for (int b = 0; b < nb && e == hipSuccess; ++b)
    {
       e = set_kpar(h, bs[b].kx, bs[b].ky);
       if (e != hipSuccess) { break; }
-      const LocalParams P = local_params(h, bs[b].mcc, bs[b].mm, apply_te, h->Ae);
+      double2 *ae = (multi && b > 0) ? h->kAe + (size_t)(b - 1) * ae_elems : h->Ae;
+      const LocalParams P = local_params(h, bs[b].mcc, bs[b].mm, apply_te, ae);
       e = enqueue_local(h, P, 0, false);
-      if (e != hipSuccess) { break; }
+      if (e != hipSuccess || multi) { continue; }
+      // this term's CSR values into its own buffer (the handle's value
+      // array swapped for it)
       h->vals = h->kbasis + (size_t)b * nnz;
       e = enqueue_scatter(h, eliminate, false);
       h->vals = vals;
    }
-   if (e == hipSuccess)
+   if (e == hipSuccess && multi)
+   {
+      MultiScatter M;
+      ScatterParams &S = M.S;
+      S.n = h->height;
+      S.nd = h->nd;
+      S.Ip = h->Ip;
+      S.Jc = h->Jc;
+      S.eflag = h->eflag;
+      S.emask = h->emask;
+      S.rc_ptr = h->rc_ptr;
+      S.rc_list = h->rc_list;
+      S.is_ess = h->ess;
+      S.Ae = h->Ae;
+      S.lda = h->lda;
+      S.vals = nullptr;
+      S.eliminate = eliminate ? 1 : 0;
+      S.maxrow = h->maxrow;
+      S.slot = h->slot;
+      S.sld = h->sld;
+      S.rsoff = h->rsoff;
+      S.rslot = h->rslot;
+      S.rows = nullptr;
+      S.nrows = h->height;
+      M.nang = nang;
+      for (int q = 0; q < 6; ++q)
+      {
+         const int qq = std::min(q, nb - 1);
+         M.Ae[q] = qq == 0 ? h->Ae : h->kAe + (size_t)(qq - 1) * ae_elems;
+      }
+      M.coef = h->d_kcoef;
+      M.out = h->ang_vals;
+      M.nnz = (int64_t)nnz;
+      const void *kf = nb == 1 ? (const void *)k_scatter_multi<1, 4>
+                       : nb == 2 ? (const void *)k_scatter_multi<2, 4>
+                       : nb == 3 ? (const void *)k_scatter_multi<3, 4>
+                       : nb == 4 ? (const void *)k_scatter_multi<4, 2>
+                       : nb == 5 ? (const void *)k_scatter_multi<5, 2>
+                                 : (const void *)k_scatter_multi<6, 2>;
+      e = hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem_multi);
+      if (e == hipSuccess)
+      {
+         void *margs[] = {&M};
+         e = hipLaunchKernel(kf, dim3((unsigned)((h->height + 3) / 4)), dim3(BLOCK), margs, shmem_multi,
+                             h->stream);
+      }
+      if (e == hipSuccess) { e = hipGetLastError(); }
+   }
+   if (e == hipSuccess && !multi)
    {
       KparCombine K;
       K.nnz = (int64_t)nnz;
@@ -2663,7 +2728,7 @@ int lfasm_angles_assemble(lfasm_handle *h, int nang, const double *kxy, int mask
       hipLaunchKernelGGL(k_kpar_combine, dim3(std::max(1u, grid)), dim3(256), 0, h->stream, K);
       e = hipGetLastError();
    }
-   if (e == hipSuccess && eliminate)
+   if (e == hipSuccess && eliminate && !multi)
    {
       hipLaunchKernelGGL(k_kpar_ess, dim3((unsigned)std::min(8192, (h->height + 255) / 256)), dim3(256),
                          0, h->stream, h->height, (const int *)h->Ip, (const unsigned char *)h->ess,

@@ -439,6 +439,128 @@ __global__ void __launch_bounds__(256) k_kpar_combine(KparCombine K)
    }
 }
 
+// The basis matrices' scatter and the combination in one pass: each
+// wavefront forms the row of all NB element-matrix sets at once (one LDS row
+// buffer per set, contributions in contributor order, as scatter_row), then
+// writes every angle's row as the combination of the NB row sums (the
+// arithmetic of k_kpar_combine), so neither the basis CSR values nor their
+// second read reach HBM.  Same bits as scatter + k_kpar_combine +
+// k_kpar_ess.  nd <= 64 (per-element bit masks), byte slot blocks.
+struct MultiScatter {
+   ScatterParams S;
+   int nang;
+   const double2 *Ae[6];
+   const double *coef;   // nang x NB
+   double2 *out;         // nang x nnz
+   int64_t nnz;
+};
+
+template <int NB, int CG>
+__global__ void __launch_bounds__(BLOCK) k_scatter_multi(MultiScatter M)
+{
+   extern __shared__ double2 rowbuf[];
+   const ScatterParams &S = M.S;
+   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   const int r = blockIdx.x * 4 + wave;
+   if (r >= S.n) { return; }
+   const int ld = S.maxrow;
+   double2 *buf = rowbuf + (size_t)wave * NB * ld;   // set q at buf + q * ld
+   const int base = S.Ip[r];
+   const int len = S.Ip[r + 1] - base;
+   const int nd = S.nd;
+#pragma unroll
+   for (int q = 0; q < NB; ++q)
+   {
+      for (int s = lane; s < len; s += 64) { buf[q * ld + s] = make_double2(0.0, 0.0); }
+   }
+   wave_sync_lds();
+   const bool row_ess = S.eliminate && S.is_ess[r];
+   const unsigned ess_bit = S.eliminate ? 2u : 0u;
+   if (!row_ess)
+   {
+      const int c0 = S.rc_ptr[r], c1 = S.rc_ptr[r + 1];
+      const uint8_t *sblk = S.rslot + S.rsoff[r];
+      const int b = lane;
+      const int bc = min(b, nd - 1);   // lanes past the row load a valid entry, unused
+      for (int cg = c0; cg < c1; cg += CG)
+      {
+         const int m = min(CG, c1 - cg);
+         int ea[CG];
+#pragma unroll
+         for (int u = 0; u < CG; ++u) { ea[u] = S.rc_list[cg + min(u, m - 1)]; }
+         dbl2v x[NB][CG];
+#pragma unroll
+         for (int q = 0; q < NB; ++q)
+         {
+#pragma unroll
+            for (int u = 0; u < CG; ++u)
+            {
+               x[q][u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2v *>(M.Ae[q]) +
+                                                    (size_t)ea[u] * S.lda + bc);
+            }
+         }
+         int sl[CG];
+         unsigned fa[CG], fb[CG];
+#pragma unroll
+         for (int u = 0; u < CG; ++u)
+         {
+            sl[u] = sblk[(size_t)(cg - c0 + min(u, m - 1)) * nd + bc];
+            const int e = ea[u] / nd;
+            const uint64_t sm = S.emask[2 * (size_t)e], em = S.emask[2 * (size_t)e + 1];
+            fa[u] = (unsigned)((sm >> (ea[u] - e * nd)) & 1);
+            fb[u] = (unsigned)(((sm >> bc) & 1) | (((em >> bc) & 1) << 1));
+         }
+#pragma unroll
+         for (int u = 0; u < CG; ++u)
+         {
+            if (u < m)
+            {
+               if (b < nd && (fb[u] & ess_bit) == 0)
+               {
+                  const bool neg = ((fa[u] ^ fb[u]) & 1) != 0;
+#pragma unroll
+                  for (int q = 0; q < NB; ++q)
+                  {
+                     double2 t = buf[q * ld + sl[u]];
+                     t.x += neg ? -x[q][u].x : x[q][u].x;
+                     t.y += neg ? -x[q][u].y : x[q][u].y;
+                     buf[q * ld + sl[u]] = t;
+                  }
+               }
+               wave_sync_lds();
+            }
+         }
+      }
+   }
+   for (int j = 0; j < M.nang; ++j)
+   {
+      const double *c = M.coef + (size_t)j * NB;
+      for (int s = lane; s < len; s += 64)
+      {
+         double re, im;
+         if (row_ess)
+         {
+            re = S.Jc[base + s] == r ? 1.0 : 0.0;
+            im = 0.0;
+         }
+         else
+         {
+            re = buf[s].x * c[0];
+            im = buf[s].y * c[0];
+#pragma unroll
+            for (int q = 1; q < NB; ++q)
+            {
+               re = __builtin_fma(c[q], buf[q * ld + s].x, re);
+               im = __builtin_fma(c[q], buf[q * ld + s].y, im);
+            }
+         }
+         __builtin_nontemporal_store(dbl2v{re, im},
+                                     reinterpret_cast<dbl2v *>(M.out + (size_t)j * M.nnz + base + s));
+      }
+   }
+}
+
 // Eliminated rows: every basis matrix holds the identity row there, so the
 // combination is replaced by B0's row (0 ... 1 ... 0) in every angle.
 __global__ void k_kpar_ess(int n, const int *__restrict__ Ip, const unsigned char *__restrict__ ess,

@@ -165,6 +165,20 @@ def test_angle_sweep_matches_plain_assembly(name, p):
                 err = maxnorm_err(got[j][0] + 1j * got[j][1], want[0] + 1j * want[1])
                 print(f"  eliminate {elim} angle {j}: vs plain {err:.2e}", flush=True)
                 assert err < 1e-14, (elim, j, err)
+        # the fused one-pass form (tensor path: basis element matrices in
+        # separate sets, scatter and combination in one kernel) and the form
+        # through each term's CSR values (taken with the fused tensor-row
+        # scatter, mode 2, which gives the tensor path's element values) agree
+        # bit for bit
+        if g.set_tensor_path(1) > 0:
+            g.angles_assemble(kxy, eliminate=True)
+            one = [g.download_angle(j) for j in range(len(kxy))]
+            g.set_tensor_path(2)
+            g.angles_assemble(kxy, eliminate=True)
+            two = [g.download_angle(j) for j in range(len(kxy))]
+            for (a0, a1), (b0, b1) in zip(one, two):
+                assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+            g.set_tensor_path(1)
         # the handle's own k_par is restored: a plain assembly matches its
         # own fresh handle bit for bit
         g.compute_local()
