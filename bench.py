@@ -29,7 +29,9 @@ work on the whole mesh over all usable host threads (at most 16).  Unless
 assembly against 64 sequential plain assemblies, order 2, 271k tets) and
 `detail.adjoint_ratio` (forward + adjoint eps gradient against the forward
 assembly + host solve, with one central-difference check) carry the other
-two parts of the metric.
+two parts of the metric; `detail.angle_sweep_speedup` is the 8-angle x 2-
+polarisation sweep (matrices from the k_par basis terms + one RHS launch)
+against per-angle assembly on the same mesh.
 """
 
 from __future__ import annotations
@@ -472,6 +474,9 @@ def main() -> int:
         a = benchmarks.adjoint(log, n_fd=1, region_check=False)
         out["detail"]["adjoint_ratio"] = a["value"]
         out["detail"]["adjoint"] = a["detail"]
+        r = benchmarks.rhs(log)
+        out["detail"]["angle_sweep_speedup"] = r["value"]
+        out["detail"]["angle_sweep"] = r["detail"]
     print(json.dumps(out), flush=True)
     return 0
 
