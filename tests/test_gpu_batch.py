@@ -123,6 +123,36 @@ def test_eps_variants_equal_plain_assembly(p):
         g.close()
 
 
+def test_angle_sweep_refusals():
+    c = _hole(3, 30.0, 6.0)
+    ad = build_asm_data(c.mesh, 2)
+    I, J = csr_symbolic(ad.dof, ad.height)
+    g = gpu.AsmGpuGlobal(0)
+    try:
+        g.init(ad, c.problem)
+        with pytest.raises(gpu.AsmGpuError) as ei:          # before build_global
+            g.angles_assemble([[0.1, 0.0]])
+        assert ei.value.code == gpu.E_STATE
+        g.build_global(I, J, ad.ess)
+        with pytest.raises(gpu.AsmGpuError) as ei:
+            g.angles_assemble(np.zeros((0, 2)))
+        assert ei.value.code == gpu.E_ARG
+        with pytest.raises(gpu.AsmGpuError):
+            g.download_angle(0)                              # no sweep yet
+    finally:
+        g.close()
+    b = gpu.AsmGpuGlobal(0)
+    try:
+        b.init_batch([(ad, c.problem), (ad, c.problem)])
+        Iu, Ju = _union_pattern([(I, J, ad.height), (I, J, ad.height)])
+        b.build_global(Iu, Ju, np.concatenate([ad.ess, ad.ess + ad.height]))
+        with pytest.raises(gpu.AsmGpuError) as ei:          # one mesh per handle
+            b.angles_assemble([[0.1, 0.0]])
+        assert ei.value.code == gpu.E_UNSUPPORTED
+    finally:
+        b.close()
+
+
 def _at_kpar(pb, kx, ky):
     q = copy.deepcopy(pb)
     q.kx, q.ky = kx, ky
