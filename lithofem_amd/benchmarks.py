@@ -172,11 +172,12 @@ def variants(log, nvar=64) -> dict:
     t_bat = _sync_time(batched, reps=2)
     t_kernel = g.timings()["batch_ms"]
     # phases of one batched run (each call returns after its work is done)
+    import torch
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     g.set_eps_elem(None)
     g.compute_local()
     g.scatter_eliminate(True)
-    import torch
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     g.variants_prepare(params)

@@ -2302,8 +2302,9 @@ namespace {
 // Host-to-device copy of pageable memory through the handle's two pinned
 // 64 MB buffers: the host copy of chunk k+1 (threads) overlaps the DMA of
 // chunk k.  Ordered on the handle's stream.
-int upload_pinned(lfasm_handle *h, void *dst, const void *src, size_t bytes)
+int upload_pinned(lfasm_handle *h, void *dst, const void *src, size_t bytes, hipStream_t st = nullptr)
 {
+   if (!st) { st = h->stream; }
    constexpr size_t CH = ((size_t)1 << 22) * sizeof(double2);
    for (int k = 0; k < 2; ++k)
    {
@@ -2327,9 +2328,8 @@ int upload_pinned(lfasm_handle *h, void *dst, const void *src, size_t bytes)
          });
       }
       for (auto &th : pool) { th.join(); }
-      LF_HIP(hipMemcpyAsync(reinterpret_cast<char *>(dst) + off, pin, len, hipMemcpyHostToDevice,
-                            h->stream));
-      LF_HIP(hipEventRecord(h->pin_ev[c & 1], h->stream));
+      LF_HIP(hipMemcpyAsync(reinterpret_cast<char *>(dst) + off, pin, len, hipMemcpyHostToDevice, st));
+      LF_HIP(hipEventRecord(h->pin_ev[c & 1], st));
    }
    return LFASM_OK;
 }
@@ -2479,10 +2479,6 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
       LF_HIP(hipMalloc((void **)&h->d_depsT, ndeps * sizeof(double2)));
       h->d_deps_cap = ndeps;
    }
-   {
-      const int rc = upload_pinned(h, h->d_deps, deps, ndeps * sizeof(double2));
-      if (rc != LFASM_OK) { return rc; }
-   }
    const double2 *d_deps = h->d_deps;
    VariantParams V;
    V.n = h->height;
@@ -2529,17 +2525,26 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
                                 : (const void *)k_variant_rows_acc<4>;
    hipError_t e = hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
    if (e == hipSuccess) { e = ev_begin(h, T_BATCH); }
+   // the copy of parameter-free runs needs no eps offsets: it runs while
+   // the offsets go up on the second stream
+   if (e == hipSuccess && h->nvfix > 0)
+   {
+      hipLaunchKernelGGL(k_variant_copy, dim3((unsigned)h->nvfix), dim3(256), 0, h->stream,
+                         (const int2 *)h->vfix, (const double2 *)h->vals, h->var_vals, h->nnz, nvar);
+      e = hipGetLastError();
+   }
+   LF_HIP(e);
+   {
+      const int rc = upload_pinned(h, h->d_deps, deps, ndeps * sizeof(double2), h->stream2);
+      if (rc != LFASM_OK) { return rc; }
+   }
+   LF_HIP(hipEventRecord(h->ev_join, h->stream2));
+   LF_HIP(hipStreamWaitEvent(h->stream, h->ev_join, 0));
    if (e == hipSuccess && vk != 3)
    {
       const dim3 tg((unsigned)((h->nparam + 31) / 32), (unsigned)((nvar + 31) / 32));
       hipLaunchKernelGGL(k_transpose_deps, tg, dim3(256), 0, h->stream, d_deps, h->d_depsT, nvar,
                          h->nparam);
-      e = hipGetLastError();
-   }
-   if (e == hipSuccess && h->nvfix > 0)
-   {
-      hipLaunchKernelGGL(k_variant_copy, dim3((unsigned)h->nvfix), dim3(256), 0, h->stream,
-                         (const int2 *)h->vfix, (const double2 *)h->vals, h->var_vals, h->nnz, nvar);
       e = hipGetLastError();
    }
    if (e == hipSuccess)
