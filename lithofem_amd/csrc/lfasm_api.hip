@@ -2517,8 +2517,15 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
    const size_t shmem = vk == 1 ? (size_t)4 * (h->maxrow + 64) * sizeof(double2)
                                 : (size_t)4 * 2 * h->maxrow * sizeof(double2);
    const int ns = (h->maxrow + 63) / 64;
+   // the accumulating form holds at most 8 contributors per row in fewer
+   // registers when the mesh allows (more wavefronts per SIMD)
+   const bool nc8 = h->pmaxc <= 8;
    const void *kfun = vk == 3 ? (const void *)k_variant_rows
                       : vk == 2 ? (const void *)k_variant_rows_reg
+                      : nc8 ? (ns <= 1 ? (const void *)k_variant_rows_acc<1, 8>
+                               : ns == 2 ? (const void *)k_variant_rows_acc<2, 8>
+                               : ns == 3 ? (const void *)k_variant_rows_acc<3, 8>
+                                         : (const void *)k_variant_rows_acc<4, 8>)
                       : ns <= 1 ? (const void *)k_variant_rows_acc<1>
                       : ns == 2 ? (const void *)k_variant_rows_acc<2>
                       : ns == 3 ? (const void *)k_variant_rows_acc<3>
@@ -2552,11 +2559,12 @@ int lfasm_variants_assemble(lfasm_handle *h, int nvar, const double *deps, int e
       const dim3 grid((unsigned)((h->height + 3) / 4));
       if (vk == 3) { hipLaunchKernelGGL(k_variant_rows, grid, dim3(BLOCK), shmem, h->stream, V); }
       else if (vk == 2) { hipLaunchKernelGGL(k_variant_rows_reg, grid, dim3(BLOCK), shmem, h->stream, V); }
-      else if (ns <= 1) { hipLaunchKernelGGL(k_variant_rows_acc<1>, grid, dim3(BLOCK), shmem, h->stream, V); }
-      else if (ns == 2) { hipLaunchKernelGGL(k_variant_rows_acc<2>, grid, dim3(BLOCK), shmem, h->stream, V); }
-      else if (ns == 3) { hipLaunchKernelGGL(k_variant_rows_acc<3>, grid, dim3(BLOCK), shmem, h->stream, V); }
-      else { hipLaunchKernelGGL(k_variant_rows_acc<4>, grid, dim3(BLOCK), shmem, h->stream, V); }
-      e = hipGetLastError();
+      else
+      {
+         void *vargs[] = {&V};
+         e = hipLaunchKernel(kfun, grid, dim3(BLOCK), vargs, shmem, h->stream);
+      }
+      if (e == hipSuccess) { e = hipGetLastError(); }
    }
    if (e == hipSuccess) { e = ev_end(h, T_BATCH); }
    if (e == hipSuccess) { e = hipStreamSynchronize(h->stream); }

@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_reg(VariantParams V)
 // own past the row (64 sink entries per wave).
 typedef const __attribute__((address_space(4))) double *const_dbl_ptr;
 
-template <int NS>
+template <int NS, int NC = VMAXC>
 __global__ void __launch_bounds__(BLOCK) k_variant_rows_acc(VariantParams V)
 {
    extern __shared__ double2 vbuf[];
@@ -252,13 +252,13 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_acc(VariantParams V)
       return;
    }
    const int nd = V.nd;
-   const int nc = c1 - c0;                    // <= VMAXC (host-checked)
+   const int nc = c1 - c0;                    // <= NC (host-checked)
    const unsigned char ess_bit = V.eliminate ? 2 : 0;
    const int b = lane;
-   double2 m[VMAXC];
-   int sl[VMAXC], pidx[VMAXC];
+   double2 m[NC];
+   int sl[NC], pidx[NC];
 #pragma unroll
-   for (int k = 0; k < VMAXC; ++k)
+   for (int k = 0; k < NC; ++k)
    {
       m[k] = make_double2(0.0, 0.0);
       sl[k] = V.maxrow + lane;
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(BLOCK) k_variant_rows_acc(VariantParams V)
       // contributors in groups of 4: the group's offsets are loaded together
       // (pidx = 0 past the row's contributors keeps the addresses valid)
 #pragma unroll
-      for (int k4 = 0; k4 < VMAXC; k4 += 4)
+      for (int k4 = 0; k4 < NC; k4 += 4)
       {
          if (k4 < nc)
          {
