@@ -164,7 +164,7 @@ def _at_kpar(pb, kx, ky):
 def test_angle_sweep_matches_plain_assembly(name, p):
     # k_par polynomial: every angle's matrix from 3 (phi = 0) or 6 basis
     # matrices equals its plain assembly to rounding, and the oracle at
-    # the north star's tolerance; theta = 0 (no k_par terms) included
+    # the 1e-13 parity tolerance; theta = 0 (no k_par terms) included
     c = line_grating(nx=4, ny=3, nz_scale=0.15, theta=6.0) if name == "grating" else \
         _hole(4, 44.0, 6.0, 30.0)
     pb = c.problem
